@@ -1,0 +1,163 @@
+/*
+ * mcml_oracle.h -- CPU oracle for the MCML-style triangle-mesh photon
+ * transport of Minx0512/LightPropagation.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (lightpropagation_amd/)
+ * may include, link or call this code.  It is used by tests/ as the checker,
+ * by __graft_entry__.smoke() to check one small GPU invocation, and by
+ * bench.py's cpu_baseline leg as the CPU restatement of the reference
+ * algorithm (brute-force triangle search, one photon per OpenMP thread).
+ *
+ * The oracle restates, statement by statement, the active path of
+ *   src/Simulation/MCLightPropagation.cl  (kernel, lines 1262-1606)
+ * together with the host-side binning of
+ *   src/Simulation/CalculatePropagation.cpp (ExecKernel, lines 1389-1759)
+ * and the host RNG initialisation of src/PSNR/PSNR.cpp (lines 48-80).
+ * Each function in mcml_oracle.c cites the reference lines it follows.
+ *
+ * Pinning (how each part is checked; see tests/test_oracle_*.py):
+ *   ran3 integer init + draws   -> golden vectors written by the reference's
+ *                                  own PSNR.cpp / ran3.c (tests/golden/)
+ *   ran3f float recurrence      -> tracks the integer stream within float
+ *                                  rounding for the first 2000 draws
+ *   philox4x32-10               -> Random123 published known-answer vectors
+ *   logf / cosf / acosf         -> error bound vs double-precision libm
+ *   ray-triangle search         -> double-precision brute force, known rays
+ *   Fresnel / Snell / Rsp       -> closed forms (normal incidence, TIR, |v|=1)
+ *   HG scattering               -> E[cos theta] = g, KS test, g = 0 isotropic
+ *   whole path                  -> MCML / van de Hulst slab (Rd, Tt),
+ *                                  Beer-Lambert ballistic fraction,
+ *                                  energy balance
+ *   tally normalisation         -> closed forms of src/IO/Logs.cpp:507-710
+ */
+#ifndef MCML_ORACLE_H
+#define MCML_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Random-number modes. */
+#define ORC_RNG_PHILOX 0 /* Philox4x32-10 keyed by (seed, photon index)     */
+#define ORC_RNG_RAN3F 1  /* reference float-state ran3, one stream/photon   */
+
+/* Trace event kinds (one record per loop iteration). */
+#define ORC_EV_INTERACT 0
+#define ORC_EV_TRANSMIT 1
+#define ORC_EV_REFLECT 2
+#define ORC_EV_ESCAPE 3
+
+/* misc[] counters */
+#define ORC_MISC_PHOTONS 0      /* photons run                              */
+#define ORC_MISC_STEPS 1        /* loop iterations                          */
+#define ORC_MISC_INTERACT 2     /* interaction iterations                   */
+#define ORC_MISC_BOUNDARY 3     /* boundary iterations                      */
+#define ORC_MISC_ESCAPED 4      /* photons that left with no face ahead     */
+#define ORC_MISC_CAPPED 5       /* photons stopped by the step cap          */
+#define ORC_MISC_ARZ_OUT 6      /* fixed-point weight deposited off the Arz grid */
+#define ORC_MISC_VOX_OUT 7      /* fixed-point weight deposited off the voxel grid */
+#define ORC_MISC_N 8
+
+typedef struct orc_params {
+    /* mesh: triangle t has vertices tri[9t..9t+8], normal nrm[3t..], and
+     * (inner, outer) tissue ids med[2t], med[2t+1]. */
+    int32_t ntri;
+    const float *tri;
+    const float *nrm;
+    const int32_t *med;
+    /* materials: nmuasg[4i..4i+3] = (n, mua, mus, g) of tissue i */
+    int32_t ntissue;
+    const float *nmuasg;
+    int32_t fetal_id;
+    /* source */
+    float pstart[3];
+    const float *vstart;  /* 3 * nstart */
+    const int32_t *vocc;  /* nstart */
+    int32_t nstart;
+    float poi_basis[12];  /* POI, eX, eY, eZ */
+    /* model extent and detection grid (x, y, z, r, phi, rho) */
+    float model_dims[3];
+    float det_grid[6];
+    /* transport */
+    float factor;
+    float chance;
+    float w_th;
+    int32_t rng_mode;
+    uint64_t seed;
+    int64_t max_steps; /* <= 0: unlimited */
+    /* extra tallies (no reference counterpart) */
+    float vox_origin[3];
+    float vox_size[3];
+    int32_t vox_n[3];  /* any zero disables the voxel grid */
+    float rad_dr;
+    int32_t rad_n;     /* 0 disables the radial reflectance tally */
+    int32_t ndet;
+    const float *det_rlo; /* annular detectors on the exit radius */
+    const float *det_rhi;
+} orc_params;
+
+/* Derived sizes, computed the way the reference host computes them
+ * (CalculatePropagation.cpp:120-243). */
+typedef struct orc_layout {
+    int32_t dimx, dimy, dimz, dimR, dimAlpha;
+    int32_t sft_nx, sft_ny;   /* SFT image: x in [-dimx/2, dimx/2] ...   */
+    int32_t arz_nr, arz_nz;   /* Arz image: r in [-dimR, dimR], z in [0, dimz] */
+    float dist_min;           /* R/T classification distance (.cl:1351)  */
+    float rsp;                /* specular reflectance (.cl:273-299)      */
+    int64_t n_sft, n_rtd, n_arz, n_region, n_vox, n_rad, n_det;
+} orc_layout;
+
+typedef struct orc_tallies {
+    int64_t *sft;    /* sft_ny * sft_nx * 3  (R, F, T), 2^32 per unit weight */
+    int64_t *rtd;    /* dimAlpha * dimR * 3 */
+    int64_t *arz;    /* arz_nz * arz_nr, 2^32 per unit weight                */
+    int64_t *phirz;  /* arz_nz * arz_nr, 2^24 per unit (weight / mua)        */
+    int64_t *region; /* ntissue */
+    int64_t *vox;    /* vox_n[2] * vox_n[1] * vox_n[0] */
+    int64_t *rad;    /* rad_n * 3 */
+    int64_t *det;    /* ndet * 2  (R, F) */
+    int64_t misc[ORC_MISC_N];
+} orc_tallies;
+
+/* ---- building blocks exposed for the tests ---- */
+float orc_logf(float x);
+float orc_cosf(float x);
+float orc_acosf(float x);
+float orc_sign(float x);
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* PSNR::InitRand3 (PSNR.cpp:48-80) with ran3Seed = idum (negative => init):
+ * fills ma[56] and returns inext / inextp after the first draw. */
+void orc_ran3_init(int64_t idum, int64_t ma[56], int32_t *inext, int32_t *inextp);
+/* PSNR::GetRan3 integer step (PSNR.cpp:105-117); returns mj. */
+int64_t orc_ran3_next(int64_t ma[56], int32_t *inext, int32_t *inextp);
+/* DoRand3 float recurrence (.cl:1204-1243). */
+float orc_ran3f_next(float s[56], int32_t *inext, int32_t *inextp);
+
+/* FindTriangleInPath (.cl:1037-1093): returns triangle id or -1, writes t. */
+int32_t orc_find_triangle(const float *tri, int32_t ntri, const float p[3],
+                          const float v[3], int32_t ref_tri, float *t_out);
+
+float orc_rspecular(const float *nmuasg, int32_t ntissue);
+float orc_fresnel(float n1, float n2, float cos_in);
+
+/* One Henyey-Greenstein deflection of v (VScatter, .cl:597-655). */
+void orc_vscatter(float v[3], float g, float xi_theta, float xi_psi);
+
+void orc_layout_compute(const orc_params *p, orc_layout *L);
+
+/* Run photons [first, first+count).  exit_rec (8 floats/photon) and the
+ * trace (12 floats/record, up to trace_max records per photon; count of
+ * records in trace_n) may be NULL.  Tallies are accumulated (not cleared).
+ * nthreads > 1 uses OpenMP.  Returns 0 on success. */
+int orc_run(const orc_params *p, int64_t first, int64_t count,
+            orc_tallies *t, float *exit_rec, float *trace, int32_t trace_max,
+            int32_t *trace_n, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,169 @@
+/*
+ * lightprop.h -- C ABI of the MI355X photon-transport library (liblp.so).
+ *
+ * Plain C types only: pointers, sizes, int status codes.  Inputs are copied
+ * in (the caller keeps ownership); outputs are copied into caller buffers.
+ * Nothing allocated by the library crosses the boundary.  A context is
+ * single-threaded; separate contexts may run concurrently.  No call throws.
+ *
+ * The reference crosses its host/device boundary through the OpenCL kernel
+ * functor of src/Simulation/CalculatePropagation.hpp:70-88 (19 arguments,
+ * invoked at CalculatePropagation.cpp:1516-1535) driven by the
+ * CalculatePropagation setters / Init() / Propagation() / Get*Image()
+ * (CalculatePropagation.hpp:212-389; call order main.cpp:174-222).  Each
+ * entry point below names the reference interface it replaces.  The C++
+ * facade lightpropagation_amd/host/CalculatePropagation.hpp keeps the
+ * reference's class interface on top of this ABI.
+ */
+#ifndef LIGHTPROP_H
+#define LIGHTPROP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  SUCCESS / FAILURE / EXPECTED_FAILURE keep the values of
+ * CalculatePropagation.cpp:23-25; the rest refine FAILURE. */
+#define LP_OK 0
+#define LP_FAILURE 1
+#define LP_EXPECTED_FAILURE 2
+#define LP_EINVAL 3      /* bad argument                         */
+#define LP_ENOMEM 4      /* device allocation failed             */
+#define LP_EDEVICE 5     /* HIP runtime / launch error           */
+#define LP_ESTATE 6      /* call out of order (e.g. no mesh yet) */
+
+/* Random-number modes (lp_set_params). */
+#define LP_RNG_PHILOX 0  /* Philox4x32-10 keyed by (seed, photon index)        */
+#define LP_RNG_RAN3 1    /* per-photon stream of the reference's float ran3     */
+
+/* Tally selectors (lp_set_tallies bit mask, lp_get_tally `which`). */
+#define LP_TALLY_SFT 0    /* exit image (R, F, T) over x,y: GetSFTImage        */
+#define LP_TALLY_RTD 1    /* exit (radius, angle) image (R, F, T): GetRTdImage */
+#define LP_TALLY_ARZ 2    /* absorbed weight over (signed r, z): GetArzImage   */
+#define LP_TALLY_PHIRZ 3  /* absorbed weight / mua over (r, z): GetPhirzImage  */
+#define LP_TALLY_REGION 4 /* absorbed weight per tissue                        */
+#define LP_TALLY_VOX 5    /* absorbed weight on a voxel grid                   */
+#define LP_TALLY_RAD 6    /* exit weight (R, F, T) over radial bins             */
+#define LP_TALLY_DET 7    /* reflected weight (R, F) per annular detector       */
+#define LP_TALLY_MISC 8   /* counters: photons, steps, interactions, boundary
+                             events, escaped, capped, off-grid Arz weight,
+                             off-grid voxel weight, BVH node visits, triangle
+                             tests, skipped searches (16 slots)               */
+#define LP_NTALLY 9
+
+/* Tally values are int64 fixed point: 2^32 per unit photon weight, except
+ * LP_TALLY_PHIRZ (2^24 per unit of weight / mua) and LP_TALLY_MISC (plain
+ * counts and 2^32 weights).  Sums are exact and independent of the order in
+ * which lanes add them. */
+#define LP_FIXED_SCALE 4294967296.0
+#define LP_FIXED_SCALE_PHI 16777216.0
+
+typedef struct lp_context lp_context;
+
+/* Sizes of the tally images, derived as the reference host derives them
+ * (InitLogVectors, CalculatePropagation.cpp:120-243). */
+typedef struct lp_layout {
+    int32_t dimx, dimy, dimz, dimR, dimAlpha;
+    int32_t sft_nx, sft_ny; /* SFT: x in [-dimx/2, dimx/2], y likewise, 3 comps */
+    int32_t arz_nr, arz_nz; /* Arz: r in [-dimR, dimR], z in [0, dimz]          */
+    float dist_min;         /* R/T classification distance (MCLightPropagation.cl:1351) */
+    float rsp;              /* specular reflectance (MCLightPropagation.cl:273-299)     */
+    int64_t n[LP_NTALLY];   /* element count of each tally                      */
+} lp_layout;
+
+typedef struct lp_stats {
+    double kernel_ms;       /* device time of the last lp_run (HIP events)      */
+    int64_t photons;        /* photons of the last lp_run                       */
+    int64_t bvh_nodes;      /* nodes of the face-search BVH                     */
+    int32_t bvh_depth;
+    int32_t grid_blocks;    /* persistent work-groups launched                  */
+    int64_t device_bytes;   /* device memory held by the context               */
+} lp_stats;
+
+/* Replaces the CalculatePropagation constructor + createContext /
+ * CreateCommandQueue (CalculatePropagation.cpp:27-80, 1225-1339). */
+int lp_create(lp_context **ctx, int32_t device);
+void lp_destroy(lp_context *ctx);
+/* Text of the last error of this context (never NULL). */
+const char *lp_last_error(const lp_context *ctx);
+
+/* Mesh: triangle t has vertices tri[9t..9t+8], normal nrm[3t..3t+2] and
+ * (inner, outer) tissue ids med[2t], med[2t+1].  Replaces SetTriangles /
+ * SetNormals / SetInnerAndOuterTissueIndices (CalculatePropagation.cpp:
+ * 541-667) and the triangles/normals/mediumIDs kernel buffers.  Builds the
+ * BVH and the safety grid on the host and uploads them. */
+int lp_set_mesh(lp_context *ctx, const float *tri, const float *nrm, const int32_t *med, int64_t ntri);
+
+/* nmuasg[4i..4i+3] = (n, mua, mus, g) of tissue i; tissue 0 is where photons
+ * start.  Replaces SetTissueProperties (cpp:427-451) + SetFetalTissueID. */
+int lp_set_materials(lp_context *ctx, const float *nmuasg, int32_t ntissue, int32_t fetal_id);
+
+/* Source: start point, nstart directions vstart[3*nstart] with photon counts
+ * vocc[nstart], and (POI, eX, eY, eZ) = poi_basis[12].  Replaces
+ * SetStartTrajectories / SetStartPoint / SetPOI / SetBasisVectors
+ * (cpp:316-350, 713-773) and the vStart, vStartOcc, pStart, e_P0XYZ args. */
+int lp_set_source(lp_context *ctx, const float pstart[3], const float *vstart, const int32_t *vocc,
+                  int32_t nstart, const float poi_basis[12]);
+
+/* Model extent (x, y, z) and detection grid (x, y, z, r, phi, rho).
+ * Replaces SetDimensions / SetDetectionGridDimensions (cpp:539-540, 706-712). */
+int lp_set_grid(lp_context *ctx, const float model_dims[3], const float det_grid[6]);
+
+/* Roulette chance, weight threshold, PixelPerUnit factor, seed, RNG mode and
+ * a per-photon step cap (<= 0: the default of 10^6).  Replaces
+ * SetChanceForRoulette / SetWeightThreshold / SetPixelPerUnitFactor
+ * (cpp:405-425) and the PSNR seeding of setup() (cpp:1893-1899). */
+int lp_set_params(lp_context *ctx, float chance, float w_th, float factor, uint64_t seed, int32_t rng_mode,
+                  int64_t max_steps);
+
+/* Tallies without a reference counterpart: voxel grid in the emitter frame
+ * (origin, voxel size, counts), radial exit bins, annular detectors on the
+ * exit radius [rlo, rhi). */
+int lp_set_voxel_grid(lp_context *ctx, const float origin[3], const float size[3], const int32_t n[3]);
+int lp_set_radial(lp_context *ctx, float dr, int32_t nbins);
+int lp_set_detectors(lp_context *ctx, const float *rlo, const float *rhi, int32_t ndet);
+
+/* Bit mask of (1 << LP_TALLY_*) to keep (MISC and REGION are always kept).
+ * record_exit != 0 keeps the per-photon float8 exit records of the next
+ * lp_run (the SurfFetalTransRxyG buffer of the reference kernel). */
+int lp_set_tallies(lp_context *ctx, uint32_t mask, int32_t record_exit);
+
+int lp_get_layout(lp_context *ctx, lp_layout *out);
+int lp_reset_tallies(lp_context *ctx);
+
+/* Transport photons [first, first + count) and add to the tallies; blocks
+ * until done.  Replaces Init() + Propagation() / ExecKernel
+ * (cpp:1389-1759, 2003-2028): one persistent launch instead of a launch,
+ * wait, copy and host binning per 6400 photons. */
+int lp_run(lp_context *ctx, int64_t first, int64_t count);
+
+/* Copy tally `which` (n elements) to out. */
+int lp_get_tally(lp_context *ctx, int32_t which, int64_t *out, int64_t n);
+
+/* Per-photon exit records of the last lp_run (8 floats each: coords(3),
+ * SFT(3), exit angle, exit radius; MCLightPropagation.cl:1586-1594). */
+int lp_get_exit_records(lp_context *ctx, float *out, int64_t nphotons);
+
+/* Runs photons [first, first + count) with a per-step event trace (12 floats
+ * per step: kind, face, tissue, w, p(3), v(3), step, draws) of up to
+ * max_rec steps each; nrec[i] receives the number of records of photon i.
+ * Tallies are accumulated as by lp_run. */
+int lp_trace(lp_context *ctx, int64_t first, int32_t count, int32_t max_rec, float *out, int32_t *nrec);
+
+int lp_get_stats(lp_context *ctx, lp_stats *out);
+
+/* Evaluates the kernel's math core element-wise on the device (op 0 sqrt,
+ * 1 divide, 2 log, 3 cos, 4 acos, 5 multiply, 6 add) over n (x, y) pairs. */
+int lp_debug_math(lp_context *ctx, int32_t op, const float *xy, float *out, int64_t n);
+
+/* Launch shape override: persistent work-groups (0 = automatic) and BVH
+ * nodes staged in LDS (-1 = automatic), safety grid cells per axis (0 =
+ * off, -1 = automatic). */
+int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32_t safety_cells);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIGHTPROP_H */

@@ -1,0 +1,217 @@
+"""ctypes binding of include/lightprop.h (liblp.so).
+
+The library is the product; this module only marshals numpy arrays across
+the C ABI.  It raises if liblp.so has not been built -- there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblp.so")
+
+LP_OK = 0
+RNG_PHILOX, RNG_RAN3 = 0, 1
+TALLY = {"sft": 0, "rtd": 1, "arz": 2, "phirz": 3, "region": 4, "vox": 5, "rad": 6, "det": 7, "misc": 8}
+MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out",
+        "node_visits", "tri_tests", "skipped"]
+FIXED = 2.0 ** 32
+FIXED_PHI = 2.0 ** 24
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+
+
+class Layout(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("dimx", "dimy", "dimz", "dimR", "dimAlpha", "sft_nx", "sft_ny",
+                                         "arz_nr", "arz_nz")] + \
+               [("dist_min", C.c_float), ("rsp", C.c_float), ("n", C.c_int64 * 9)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("photons", C.c_int64), ("bvh_nodes", C.c_int64),
+                ("bvh_depth", C.c_int32), ("grid_blocks", C.c_int32), ("device_bytes", C.c_int64)]
+
+
+class LPError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_SIGS = {
+    "lp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32]),
+    "lp_destroy": (None, [C.c_void_p]),
+    "lp_last_error": (C.c_char_p, [C.c_void_p]),
+    "lp_set_mesh": (C.c_int, [C.c_void_p, _fp, _fp, _ip, C.c_int64]),
+    "lp_set_materials": (C.c_int, [C.c_void_p, _fp, C.c_int32, C.c_int32]),
+    "lp_set_source": (C.c_int, [C.c_void_p, _fp, _fp, _ip, C.c_int32, _fp]),
+    "lp_set_grid": (C.c_int, [C.c_void_p, _fp, _fp]),
+    "lp_set_params": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_uint64, C.c_int32, C.c_int64]),
+    "lp_set_voxel_grid": (C.c_int, [C.c_void_p, _fp, _fp, _ip]),
+    "lp_set_radial": (C.c_int, [C.c_void_p, C.c_float, C.c_int32]),
+    "lp_set_detectors": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32]),
+    "lp_set_tallies": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32]),
+    "lp_get_layout": (C.c_int, [C.c_void_p, C.POINTER(Layout)]),
+    "lp_reset_tallies": (C.c_int, [C.c_void_p]),
+    "lp_run": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64]),
+    "lp_get_tally": (C.c_int, [C.c_void_p, C.c_int32, _lp, C.c_int64]),
+    "lp_get_exit_records": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
+    "lp_trace": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, _fp, _ip]),
+    "lp_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "lp_debug_math": (C.c_int, [C.c_void_p, C.c_int32, _fp, _fp, C.c_int64]),
+    "lp_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+}
+
+
+def lib():
+    """Load liblp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LPError("liblp.so not built: run python -c 'import __graft_entry__ as g; g.build()'")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def fp(a):
+    return a.ctypes.data_as(_fp)
+
+
+def ip(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Context:
+    """One lp_context on one device."""
+
+    def __init__(self, device=0):
+        self._c = C.c_void_p()
+        rc = lib().lp_create(C.byref(self._c), device)
+        if rc != LP_OK:
+            raise LPError("lp_create(device=%d) failed with %d (no HIP device?)" % (device, rc))
+        self._keep = {}
+
+    def close(self):
+        if self._c:
+            lib().lp_destroy(self._c)
+            self._c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != LP_OK:
+            raise LPError("%s failed (%d): %s" % (what, rc, lib().lp_last_error(self._c).decode()))
+
+    # --- configuration ----------------------------------------------------
+    def set_mesh(self, tri, nrm, med):
+        tri, nrm, med = _f(tri).reshape(-1), _f(nrm).reshape(-1), _i(med).reshape(-1)
+        self._chk(lib().lp_set_mesh(self._c, fp(tri), fp(nrm), ip(med), len(tri) // 9), "lp_set_mesh")
+
+    def set_materials(self, nmuasg, fetal_id):
+        nm = _f(nmuasg).reshape(-1)
+        self._chk(lib().lp_set_materials(self._c, fp(nm), len(nm) // 4, int(fetal_id)), "lp_set_materials")
+
+    def set_source(self, pstart, vstart, vocc, poi_basis):
+        ps, vs, vo, pb = _f(pstart), _f(vstart).reshape(-1), _i(vocc).reshape(-1), _f(poi_basis)
+        self._chk(lib().lp_set_source(self._c, fp(ps), fp(vs), ip(vo), len(vo), fp(pb)), "lp_set_source")
+
+    def set_grid(self, model_dims, det_grid):
+        self._chk(lib().lp_set_grid(self._c, fp(_f(model_dims)), fp(_f(det_grid))), "lp_set_grid")
+
+    def set_params(self, chance, w_th, factor=1.0, seed=1, rng=RNG_PHILOX, max_steps=0):
+        self._chk(lib().lp_set_params(self._c, chance, w_th, factor, int(seed) & (2 ** 64 - 1), rng, max_steps),
+                  "lp_set_params")
+
+    def set_voxel_grid(self, origin, size, n):
+        self._chk(lib().lp_set_voxel_grid(self._c, fp(_f(origin)), fp(_f(size)), ip(_i(n))), "lp_set_voxel_grid")
+
+    def set_radial(self, dr, n):
+        self._chk(lib().lp_set_radial(self._c, dr, n), "lp_set_radial")
+
+    def set_detectors(self, rlo, rhi):
+        lo, hi = _f(rlo), _f(rhi)
+        self._chk(lib().lp_set_detectors(self._c, fp(lo), fp(hi), len(lo)), "lp_set_detectors")
+
+    def set_tallies(self, mask=0xFFFFFFFF, record_exit=False):
+        self._chk(lib().lp_set_tallies(self._c, mask, int(bool(record_exit))), "lp_set_tallies")
+
+    def set_launch(self, grid_blocks=0, lds_nodes=-1, safety_cells=-1):
+        self._chk(lib().lp_set_launch(self._c, grid_blocks, lds_nodes, safety_cells), "lp_set_launch")
+
+    def set_scene(self, sc, seed=1, rng=RNG_PHILOX, max_steps=0):
+        """Configure from a scene dict (tri, nrm, med, nmuasg, fetal_id, pstart,
+        vstart, vocc, poi_basis, model_dims, det_grid, factor, chance, w_th)."""
+        self.set_mesh(sc["tri"], sc["nrm"], sc["med"])
+        self.set_materials(sc["nmuasg"], sc.get("fetal_id", -1))
+        self.set_source(sc["pstart"], sc["vstart"], sc["vocc"], sc["poi_basis"])
+        self.set_grid(sc["model_dims"], sc["det_grid"])
+        self.set_params(float(sc["chance"]), float(sc["w_th"]), float(sc.get("factor", 1.0)), seed, rng, max_steps)
+
+    # --- execution ----------------------------------------------------------
+    def layout(self):
+        L = Layout()
+        self._chk(lib().lp_get_layout(self._c, C.byref(L)), "lp_get_layout")
+        return L
+
+    def reset(self):
+        self._chk(lib().lp_reset_tallies(self._c), "lp_reset_tallies")
+
+    def run(self, first, count):
+        self._chk(lib().lp_run(self._c, int(first), int(count)), "lp_run")
+
+    def tally(self, name):
+        k = TALLY[name]
+        n = self.layout().n[k]
+        out = np.zeros(max(n, 0), np.int64)
+        self._chk(lib().lp_get_tally(self._c, k, out.ctypes.data_as(_lp), n), "lp_get_tally")
+        return out
+
+    def tallies(self):
+        out = {k: self.tally(k) for k in TALLY if k != "misc"}
+        m = self.tally("misc")
+        out["misc"] = dict(zip(MISC, [int(x) for x in m[:len(MISC)]]))
+        return out
+
+    def exit_records(self, n):
+        out = np.zeros((n, 8), np.float32)
+        self._chk(lib().lp_get_exit_records(self._c, fp(out), n), "lp_get_exit_records")
+        return out
+
+    def trace(self, first, count, max_rec):
+        out = np.zeros((count, max_rec, 12), np.float32)
+        nrec = np.zeros(count, np.int32)
+        self._chk(lib().lp_trace(self._c, first, count, max_rec, fp(out), ip(nrec)), "lp_trace")
+        return out, nrec
+
+    def stats(self):
+        s = Stats()
+        self._chk(lib().lp_get_stats(self._c, C.byref(s)), "lp_get_stats")
+        return s
+
+    def debug_math(self, op, x, y=None):
+        x = _f(x)
+        y = np.zeros_like(x) if y is None else _f(y)
+        xy = np.ascontiguousarray(np.stack([x, y], axis=1), np.float32)
+        out = np.zeros(len(x), np.float32)
+        self._chk(lib().lp_debug_math(self._c, op, fp(xy), fp(out), len(x)), "lp_debug_math")
+        return out

@@ -1,0 +1,319 @@
+// lp_bvh.cpp -- host-side acceleration structures for the triangle search.
+//
+// The reference searches all T triangles for every step
+// (FindTriangleInPath, src/Simulation/MCLightPropagation.cl:1037-1093).  The
+// kernel here answers the same query -- nearest face along the ray, strict
+// lowest-index tie-break, refTri excluded -- through a flattened binary BVH
+// built below, and skips the search entirely for steps whose length is below
+// a conservative distance-to-mesh bound (the safety grid).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <numeric>
+
+#include "lp_internal.hpp"
+
+namespace lp {
+namespace {
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const float* p)
+    {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], p[a]);
+            hi[a] = std::max(hi[a], p[a]);
+        }
+    }
+    void grow(const Box& b)
+    {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], b.lo[a]);
+            hi[a] = std::max(hi[a], b.hi[a]);
+        }
+    }
+    double area() const
+    {
+        double d[3];
+        for (int a = 0; a < 3; ++a) d[a] = std::max(0.0, (double)hi[a] - lo[a]);
+        return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+    }
+};
+
+struct TNode {
+    Box box;
+    int32_t child[2] = {-1, -1};
+    int32_t first = 0, count = 0;  // leaf if count > 0
+    int32_t depth = 0;
+};
+
+}  // namespace
+
+Bvh build_bvh(const float* tri, const int32_t* med, int64_t ntri, int max_leaf)
+{
+    Bvh out;
+    max_leaf = std::min(std::max(max_leaf, 1), kMaxLeaf / 2);
+    std::vector<Box> tb((size_t)ntri);
+    std::vector<float> cen((size_t)ntri * 3);
+    Box scene;
+    for (int64_t t = 0; t < ntri; ++t) {
+        for (int v = 0; v < 3; ++v) tb[t].grow(tri + 9 * t + 3 * v);
+        for (int a = 0; a < 3; ++a) cen[3 * t + a] = 0.5f * (tb[t].lo[a] + tb[t].hi[a]);
+        scene.grow(tb[t]);
+    }
+    double diag = 0, maxabs = 0;
+    for (int a = 0; a < 3; ++a) {
+        double d = (double)scene.hi[a] - scene.lo[a];
+        diag += d * d;
+        maxabs = std::max({maxabs, std::fabs((double)scene.hi[a]), std::fabs((double)scene.lo[a])});
+    }
+    diag = std::sqrt(diag);
+    out.pad = (float)std::max(1e-5 * diag, 4e-6 * maxabs) + 1e-12f;
+
+    std::vector<int32_t> idx((size_t)ntri);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::vector<TNode> nodes;
+    nodes.reserve((size_t)std::max<int64_t>(1, 2 * ntri / std::max(1, max_leaf) + 4));
+    TNode root;
+    root.first = 0;
+    root.count = (int32_t)ntri;
+    for (int64_t t = 0; t < ntri; ++t) root.box.grow(tb[t]);
+    nodes.push_back(root);
+
+    const int kBins = 16;
+    const int kMaxDepth = 56;
+    std::vector<int32_t> work{0};
+    while (!work.empty()) {
+        int32_t ni = work.back();
+        work.pop_back();
+        TNode n = nodes[ni];
+        if (n.count <= max_leaf) continue;
+        Box cb;
+        for (int32_t i = n.first; i < n.first + n.count; ++i) cb.grow(&cen[3 * (size_t)idx[i]]);
+        int best_axis = -1, best_split = -1;
+        double best_cost = INFINITY;
+        for (int a = 0; a < 3; ++a) {
+            double ext = (double)cb.hi[a] - cb.lo[a];
+            if (!(ext > 0)) continue;
+            Box bb[kBins];
+            int cnt[kBins] = {0};
+            for (int32_t i = n.first; i < n.first + n.count; ++i) {
+                int b = (int)((cen[3 * (size_t)idx[i] + a] - cb.lo[a]) / ext * kBins);
+                b = std::min(std::max(b, 0), kBins - 1);
+                cnt[b]++;
+                bb[b].grow(tb[idx[i]]);
+            }
+            double la[kBins], ra[kBins];
+            int lc[kBins], rc[kBins];
+            Box acc;
+            int c = 0;
+            for (int b = 0; b < kBins; ++b) {
+                acc.grow(bb[b]);
+                c += cnt[b];
+                la[b] = acc.area();
+                lc[b] = c;
+            }
+            acc = Box();
+            c = 0;
+            for (int b = kBins - 1; b >= 0; --b) {
+                acc.grow(bb[b]);
+                c += cnt[b];
+                ra[b] = acc.area();
+                rc[b] = c;
+            }
+            for (int b = 0; b < kBins - 1; ++b) {
+                if (lc[b] == 0 || rc[b + 1] == 0) continue;
+                double cost = la[b] * lc[b] + ra[b + 1] * rc[b + 1];
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = a;
+                    best_split = b;
+                }
+            }
+        }
+        int32_t mid;
+        double leaf_cost = n.box.area() * n.count;
+        bool force = n.depth >= kMaxDepth - 1;
+        if (best_axis < 0 || force) {
+            // median split along the longest centroid axis (ties by index)
+            int a = 0;
+            for (int k = 1; k < 3; ++k)
+                if (cb.hi[k] - cb.lo[k] > cb.hi[a] - cb.lo[a]) a = k;
+            std::sort(idx.begin() + n.first, idx.begin() + n.first + n.count, [&](int32_t x, int32_t y) {
+                float cx = cen[3 * (size_t)x + a], cy = cen[3 * (size_t)y + a];
+                return cx < cy || (cx == cy && x < y);
+            });
+            mid = n.first + n.count / 2;
+        } else {
+            if (n.count <= 16 && best_cost + 0.5 * n.box.area() >= leaf_cost) {
+                if (n.count <= 2 * max_leaf) continue;  // keep as leaf
+            }
+            double ext = (double)cb.hi[best_axis] - cb.lo[best_axis];
+            auto it = std::partition(idx.begin() + n.first, idx.begin() + n.first + n.count, [&](int32_t x) {
+                int b = (int)((cen[3 * (size_t)x + best_axis] - cb.lo[best_axis]) / ext * kBins);
+                b = std::min(std::max(b, 0), kBins - 1);
+                return b <= best_split;
+            });
+            mid = (int32_t)(it - idx.begin());
+            if (mid == n.first || mid == n.first + n.count) mid = n.first + n.count / 2;
+        }
+        TNode L, R;
+        L.first = n.first;
+        L.count = mid - n.first;
+        R.first = mid;
+        R.count = n.first + n.count - mid;
+        L.depth = R.depth = n.depth + 1;
+        for (int32_t i = L.first; i < L.first + L.count; ++i) L.box.grow(tb[idx[i]]);
+        for (int32_t i = R.first; i < R.first + R.count; ++i) R.box.grow(tb[idx[i]]);
+        int32_t li = (int32_t)nodes.size();
+        nodes.push_back(L);
+        nodes.push_back(R);
+        nodes[ni].child[0] = li;
+        nodes[ni].child[1] = li + 1;
+        nodes[ni].count = 0;
+        work.push_back(li + 1);
+        work.push_back(li);
+    }
+
+    // Flatten: every tree node becomes a threaded BvhNode, numbered breadth
+    // first so the top levels are contiguous (and can be staged in LDS).
+    std::vector<int32_t> order;
+    std::vector<int32_t> flat_of(nodes.size(), -1);
+    std::deque<int32_t> q;
+    if (ntri > 0) q.push_back(0);
+    while (!q.empty()) {
+        int32_t t = q.front();
+        q.pop_front();
+        flat_of[t] = (int32_t)order.size();
+        order.push_back(t);
+        if (nodes[t].count == 0)
+            for (int c = 0; c < 2; ++c) q.push_back(nodes[t].child[c]);
+    }
+    std::vector<int32_t> miss_of(nodes.size(), -1);
+    for (int32_t t : order) {  // parents precede children in BFS order
+        if (nodes[t].count == 0) {
+            miss_of[nodes[t].child[0]] = flat_of[nodes[t].child[1]];
+            miss_of[nodes[t].child[1]] = miss_of[t];
+        }
+    }
+    out.tris.resize((size_t)ntri);
+    for (int64_t i = 0; i < ntri; ++i) {
+        int32_t t = idx[i];
+        const float* K = tri + 9 * (size_t)t;
+        const float* Lv = K + 3;
+        const float* M = K + 6;
+        TriRec& r = out.tris[i];
+        for (int a = 0; a < 3; ++a) {
+            r.m[a] = M[a];
+            r.e[a] = K[a] - M[a];
+            r.f[a] = Lv[a] - M[a];
+        }
+        r.id = t;
+        r.med = med ? ((med[2 * t] & 0xffff) | (med[2 * t + 1] << 16)) : 0;
+        r.pad = 0;
+    }
+    out.nodes.resize(order.size());
+    for (size_t k = 0; k < order.size(); ++k) {
+        const TNode& n = nodes[order[k]];
+        BvhNode& bn = out.nodes[k];
+        for (int a = 0; a < 3; ++a) {
+            bn.lo[a] = n.box.lo[a] - out.pad;
+            bn.hi[a] = n.box.hi[a] + out.pad;
+        }
+        if (n.count > 0)
+            bn.hit = (int32_t)(0x80000000u | ((uint32_t)n.count << 24) | (uint32_t)n.first);
+        else
+            bn.hit = flat_of[n.child[0]];
+        bn.miss = miss_of[order[k]];
+    }
+    int32_t depth = 0;
+    for (auto& n : nodes) depth = std::max(depth, n.depth);
+    out.depth = depth;
+    return out;
+}
+
+// Chebyshev distance transform over cells touched by triangle boxes.  For a
+// point in cell C and a triangle touching only cells at Chebyshev distance
+// >= d from C, the point-triangle distance is >= (d - 1) * min cell edge.
+SafetyGrid build_safety_grid(const Bvh& bvh, const float* tri, int64_t ntri, int cells)
+{
+    (void)bvh;
+    SafetyGrid g;
+    if (ntri == 0 || cells < 4) return g;
+    Box sb;
+    for (int64_t t = 0; t < ntri; ++t)
+        for (int v = 0; v < 3; ++v) sb.grow(tri + 9 * t + 3 * v);
+    double ext[3], mx = 0;
+    for (int a = 0; a < 3; ++a) {
+        ext[a] = (double)sb.hi[a] - sb.lo[a];
+        mx = std::max(mx, ext[a]);
+    }
+    double h = mx / cells;
+    if (!(h > 0)) return g;
+    for (int a = 0; a < 3; ++a) {
+        double lo = sb.lo[a] - 2 * h;
+        g.n[a] = (int32_t)std::ceil((ext[a] + 4 * h) / h);
+        g.origin[a] = (float)lo;
+        g.inv_cell[a] = (float)(1.0 / h);
+    }
+    const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int64_t N = nx * ny * nz;
+    const int32_t INF = 1 << 29;
+    std::vector<int32_t> d((size_t)N, INF);
+    std::vector<int64_t> frontier;
+    auto cell_of = [&](double x, int a) {
+        int64_t c = (int64_t)std::floor((x - g.origin[a]) / h);
+        return std::min<int64_t>(std::max<int64_t>(c, 0), g.n[a] - 1);
+    };
+    for (int64_t t = 0; t < ntri; ++t) {
+        Box b;
+        for (int v = 0; v < 3; ++v) b.grow(tri + 9 * t + 3 * v);
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = cell_of((double)b.lo[a] - 1e-6 * mx, a);
+            hi[a] = cell_of((double)b.hi[a] + 1e-6 * mx, a);
+        }
+        for (int64_t z = lo[2]; z <= hi[2]; ++z)
+            for (int64_t y = lo[1]; y <= hi[1]; ++y)
+                for (int64_t x = lo[0]; x <= hi[0]; ++x) {
+                    int64_t c = (z * ny + y) * nx + x;
+                    if (d[c] != 0) {
+                        d[c] = 0;
+                        frontier.push_back(c);
+                    }
+                }
+    }
+    // multi-source BFS with the 26-neighbourhood = Chebyshev distance
+    std::vector<int64_t> next;
+    for (int32_t dist = 1; !frontier.empty(); ++dist) {
+        next.clear();
+        for (int64_t c : frontier) {
+            int64_t x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int64_t X = x + dx, Y = y + dy, Z = z + dz;
+                        if (X < 0 || Y < 0 || Z < 0 || X >= nx || Y >= ny || Z >= nz) continue;
+                        int64_t k = (Z * ny + Y) * nx + X;
+                        if (d[k] > dist) {
+                            d[k] = dist;
+                            next.push_back(k);
+                        }
+                    }
+        }
+        frontier.swap(next);
+    }
+    // safety = (d - 1) * h, shrunk by 2% and a scene-relative margin that
+    // covers the rounding of the float triangle test
+    double margin = 1e-5 * std::sqrt(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
+    g.dist.resize((size_t)N);
+    for (int64_t c = 0; c < N; ++c) {
+        double s = (d[c] >= INF) ? 0.0 : (d[c] - 1) * h * 0.98 - margin;
+        g.dist[c] = (float)std::max(0.0, s);
+    }
+    return g;
+}
+
+}  // namespace lp

@@ -1,0 +1,644 @@
+// lp_capi.cpp -- host runtime behind include/lightprop.h.
+//
+// Owns the device copies of the scene (leaf-ordered triangles, threaded
+// BVH, safety grid, normals, tissue ids), the tallies and the launch.  It
+// replaces the OpenCL host runtime of src/Simulation/CalculatePropagation.cpp
+// (buffer packing :1167-1222, the 6400-photon launch/wait/copy loop with host
+// binning :1389-1759) by one persistent launch that bins on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lightprop.h"
+#include "lp_internal.hpp"
+
+using namespace lp;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct lp_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err = "";
+    int num_cus = 0;
+
+    // host state
+    std::vector<float> tri, nrm;
+    std::vector<int32_t> med;
+    int64_t ntri = -1;
+    std::vector<float> nmuasg;
+    int32_t ntissue = 0, fetal_id = -1;
+    float pstart[3] = {0, 0, 0};
+    std::vector<float> vstart;
+    std::vector<int32_t> vocc;
+    float poi_basis[12] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+    bool have_source = false, have_grid = false;
+    float model_dims[3] = {1, 1, 1}, det_grid[6] = {1, 1, 1, 1, 1, 1};
+    float chance = 0.1f, w_th = 1e-4f, factor = 1.0f;
+    uint64_t seed = 1;
+    int32_t rng_mode = 0;
+    int64_t max_steps = 1000000;
+    float vox_origin[3] = {0, 0, 0}, vox_size[3] = {1, 1, 1};
+    int32_t vox_n[3] = {0, 0, 0};
+    float rad_dr = 0.05f;
+    int32_t rad_n = 0;
+    std::vector<float> det_rlo, det_rhi;
+    uint32_t tally_mask = 0xffffffffu;
+    int32_t record_exit = 0;
+    int32_t grid_override = 0, lds_override = -1, safety_cells = -1;
+
+    // derived
+    lp_layout L{};
+    Bvh bvh;
+    SafetyGrid sg;
+    bool mesh_dirty = true, src_dirty = true, tallies_dirty = true;
+
+    // device
+    DevBuf d_tris, d_nodes, d_nrm, d_med, d_safety, d_vstart, d_vocc, d_counter, d_exit;
+    DevBuf d_tally[LP_NTALLY];
+    int64_t exit_count = 0;
+    lp_stats stats{};
+};
+
+namespace {
+
+int fail(lp_context* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) return fail((c), LP_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+int dev_alloc(lp_context* c, DevBuf& b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (b.p && b.bytes >= bytes) return LP_OK;
+    if (b.p) {
+        hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, LP_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return LP_OK;
+}
+
+void dev_free(DevBuf& b)
+{
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+int upload(lp_context* c, DevBuf& b, const void* src, size_t bytes)
+{
+    int rc = dev_alloc(c, b, bytes);
+    if (rc) return rc;
+    if (bytes) HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return LP_OK;
+}
+
+// x86 cvtt semantics of the reference host's int()
+int32_t cvt_host(double x)
+{
+    if (!(x > -2147483649.0 && x < 2147483648.0)) return INT32_MIN;
+    return (int32_t)x;
+}
+
+// normalize(p) = p * (1/sqrt(dot(p,p))) in float, as on the device
+void normalize3(const float* in, float* out)
+{
+    float d = (in[0] * in[0] + in[1] * in[1]) + in[2] * in[2];
+    if (d == 0.0f) {
+        std::memcpy(out, in, 3 * sizeof(float));
+        return;
+    }
+    float inv = 1.0f / std::sqrt(d);
+    for (int k = 0; k < 3; ++k) out[k] = inv * in[k];
+}
+
+// Rspecular (MCLightPropagation.cl:273-299)
+float rspecular(const float* nm, int32_t ntissue)
+{
+    float n1 = nm[0], n2 = nm[4];
+    float temp = (n1 - n2) / (n1 + n2);
+    float r1 = temp * temp;
+    if (nm[5] == 0.0f && nm[6] == 0.0f && ntissue > 2) {
+        float n3 = nm[8];
+        temp = (n2 - n3) / (n2 + n3);
+        float r2 = temp * temp;
+        r1 = r1 + (1 - r1) * (1 - r1) * r2 / (1 - r1 * r2);
+    }
+    return r1;
+}
+
+void compute_layout(lp_context* c)
+{
+    lp_layout& L = c->L;
+    std::memset(&L, 0, sizeof L);
+    L.dimx = cvt_host(c->model_dims[0] / c->det_grid[0]);
+    L.dimy = cvt_host(c->model_dims[1] / c->det_grid[1]);
+    L.dimz = cvt_host(c->model_dims[2] / c->det_grid[2]);
+    L.dimAlpha = cvt_host(90 / c->det_grid[5]);
+    L.dimR = (int32_t)std::ceil(std::sqrt(std::pow((double)(L.dimx / 2), 2) + std::pow((double)(L.dimy / 2), 2)));
+    L.sft_nx = 2 * (L.dimx / 2) + 1;
+    L.sft_ny = 2 * (L.dimy / 2) + 1;
+    L.arz_nr = 2 * L.dimR + 1;
+    L.arz_nz = L.dimz + 1;
+    L.dist_min = c->det_grid[2] * (float)L.dimz / 2.0f;
+    L.rsp = c->ntissue >= 2 ? rspecular(c->nmuasg.data(), c->ntissue) : 0.0f;
+    L.n[LP_TALLY_SFT] = (int64_t)L.sft_nx * L.sft_ny * 3;
+    L.n[LP_TALLY_RTD] = (L.dimR > 0 && L.dimAlpha > 0) ? (int64_t)L.dimR * L.dimAlpha * 3 : 0;
+    L.n[LP_TALLY_ARZ] = (int64_t)L.arz_nr * L.arz_nz;
+    L.n[LP_TALLY_PHIRZ] = L.n[LP_TALLY_ARZ];
+    L.n[LP_TALLY_REGION] = c->ntissue;
+    L.n[LP_TALLY_VOX] = (c->vox_n[0] > 0 && c->vox_n[1] > 0 && c->vox_n[2] > 0)
+                            ? (int64_t)c->vox_n[0] * c->vox_n[1] * c->vox_n[2] : 0;
+    L.n[LP_TALLY_RAD] = c->rad_n > 0 ? (int64_t)c->rad_n * 3 : 0;
+    L.n[LP_TALLY_DET] = (int64_t)c->det_rlo.size() * 2;
+    L.n[LP_TALLY_MISC] = kMiscN;
+}
+
+bool tally_on(const lp_context* c, int k)
+{
+    if (k == LP_TALLY_MISC || k == LP_TALLY_REGION) return true;
+    return (c->tally_mask >> k) & 1u;
+}
+
+int check_ready(lp_context* c)
+{
+    if (c->ntri < 0) return fail(c, LP_ESTATE, "no mesh (lp_set_mesh)");
+    if (c->ntissue < 2) return fail(c, LP_ESTATE, "no materials (lp_set_materials)");
+    if (!c->have_source) return fail(c, LP_ESTATE, "no source (lp_set_source)");
+    for (int64_t t = 0; t < c->ntri; ++t)
+        for (int s = 0; s < 2; ++s)
+            if (c->med[2 * t + s] < 0 || c->med[2 * t + s] >= c->ntissue)
+                return fail(c, LP_EINVAL, "triangle %lld has tissue id %d outside [0, %d)", (long long)t,
+                            c->med[2 * t + s], c->ntissue);
+    return LP_OK;
+}
+
+int prepare(lp_context* c)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->mesh_dirty) {
+        c->bvh = build_bvh(c->tri.data(), c->med.data(), c->ntri, 4);
+        int cells = c->safety_cells < 0 ? 128 : c->safety_cells;
+        c->sg = cells > 0 ? build_safety_grid(c->bvh, c->tri.data(), c->ntri, cells) : SafetyGrid();
+        if ((rc = upload(c, c->d_tris, c->bvh.tris.data(), c->bvh.tris.size() * sizeof(TriRec)))) return rc;
+        if ((rc = upload(c, c->d_nodes, c->bvh.nodes.data(), c->bvh.nodes.size() * sizeof(BvhNode)))) return rc;
+        if ((rc = upload(c, c->d_nrm, c->nrm.data(), c->nrm.size() * sizeof(float)))) return rc;
+        if ((rc = upload(c, c->d_med, c->med.data(), c->med.size() * sizeof(int32_t)))) return rc;
+        if (!c->sg.dist.empty()) {
+            if ((rc = upload(c, c->d_safety, c->sg.dist.data(), c->sg.dist.size() * sizeof(float)))) return rc;
+        }
+        c->mesh_dirty = false;
+    }
+    if (c->src_dirty) {
+        if ((rc = upload(c, c->d_vstart, c->vstart.data(), c->vstart.size() * sizeof(float)))) return rc;
+        if ((rc = upload(c, c->d_vocc, c->vocc.data(), c->vocc.size() * sizeof(int32_t)))) return rc;
+        c->src_dirty = false;
+    }
+    compute_layout(c);
+    if (c->tallies_dirty) {
+        for (int k = 0; k < LP_NTALLY; ++k) {
+            size_t bytes = (size_t)std::max<int64_t>(c->L.n[k], 1) * sizeof(int64_t);
+            if ((rc = dev_alloc(c, c->d_tally[k], bytes))) return rc;
+            HIPCHK(c, hipMemsetAsync(c->d_tally[k].p, 0, bytes, c->stream));
+        }
+        c->tallies_dirty = false;
+    }
+    if ((rc = dev_alloc(c, c->d_counter, 64))) return rc;
+    return LP_OK;
+}
+
+void fill_args(lp_context* c, KArgs& a)
+{
+    std::memset(&a, 0, sizeof a);
+    a.tris = (const TriRec*)c->d_tris.p;
+    a.nodes = (const BvhNode*)c->d_nodes.p;
+    a.nnodes = (int32_t)c->bvh.nodes.size();
+    int lds = c->lds_override >= 0 ? c->lds_override : 512;
+    a.nlds = std::min<int32_t>(a.nnodes, lds);
+    a.nrm = (const float*)c->d_nrm.p;
+    a.med = (const int32_t*)c->d_med.p;
+    a.ntri = (int32_t)c->ntri;
+    if (!c->sg.dist.empty()) {
+        a.safety = (const float*)c->d_safety.p;
+        for (int k = 0; k < 3; ++k) {
+            a.sg_origin[k] = c->sg.origin[k];
+            a.sg_inv[k] = c->sg.inv_cell[k];
+            a.sg_n[k] = c->sg.n[k];
+        }
+    }
+    std::memcpy(a.nmuasg, c->nmuasg.data(), c->nmuasg.size() * sizeof(float));
+    a.ntissue = c->ntissue;
+    a.fetal_id = c->fetal_id;
+    std::memcpy(a.pstart, c->pstart, sizeof a.pstart);
+    a.vstart = (const float*)c->d_vstart.p;
+    a.vocc = (const int32_t*)c->d_vocc.p;
+    a.nstart = (int32_t)c->vocc.size();
+    std::memcpy(a.poi, c->poi_basis, 3 * sizeof(float));
+    normalize3(c->poi_basis + 3, a.ex);
+    normalize3(c->poi_basis + 6, a.ey);
+    normalize3(c->poi_basis + 9, a.ez);
+    std::memcpy(a.ez_raw, c->poi_basis + 9, 3 * sizeof(float));
+    std::memcpy(a.det_grid, c->det_grid, sizeof a.det_grid);
+    a.rsp = c->L.rsp;
+    a.dist_min = c->L.dist_min;
+    a.dimx = c->L.dimx;
+    a.dimy = c->L.dimy;
+    a.dimz = c->L.dimz;
+    a.dimR = c->L.dimR;
+    a.dimAlpha = c->L.dimAlpha;
+    a.sft_nx = c->L.sft_nx;
+    a.sft_ny = c->L.sft_ny;
+    a.arz_nr = c->L.arz_nr;
+    a.arz_nz = c->L.arz_nz;
+    a.factor = c->factor;
+    a.chance = c->chance;
+    a.w_th = c->w_th;
+    a.seed = c->seed;
+    a.max_steps = c->max_steps;
+    a.rng_mode = c->rng_mode;
+    auto T = [&](int k) -> unsigned long long* {
+        return (tally_on(c, k) && c->L.n[k] > 0) ? (unsigned long long*)c->d_tally[k].p : nullptr;
+    };
+    a.sft = T(LP_TALLY_SFT);
+    a.rtd = T(LP_TALLY_RTD);
+    a.arz = T(LP_TALLY_ARZ);
+    a.phirz = a.arz ? (unsigned long long*)c->d_tally[LP_TALLY_PHIRZ].p : nullptr;
+    a.region = T(LP_TALLY_REGION);
+    a.vox = T(LP_TALLY_VOX);
+    a.rad = T(LP_TALLY_RAD);
+    a.det = c->det_rlo.empty() ? nullptr : T(LP_TALLY_DET);
+    a.misc = (unsigned long long*)c->d_tally[LP_TALLY_MISC].p;
+    std::memcpy(a.vox_origin, c->vox_origin, sizeof a.vox_origin);
+    std::memcpy(a.vox_size, c->vox_size, sizeof a.vox_size);
+    std::memcpy(a.vox_n, c->vox_n, sizeof a.vox_n);
+    a.rad_dr = c->rad_dr;
+    a.rad_n = c->rad_n;
+    a.ndet = (int32_t)c->det_rlo.size();
+    for (int k = 0; k < a.ndet; ++k) {
+        a.det_rlo[k] = c->det_rlo[k];
+        a.det_rhi[k] = c->det_rhi[k];
+    }
+    a.counter = (unsigned long long*)c->d_counter.p;
+}
+
+int launch(lp_context* c, KArgs& a)
+{
+    int grid = c->grid_override;
+    if (grid <= 0) {
+        // enough work-groups to fill every CU at the occupancy the kernel allows
+        int per_cu = 4;
+        grid = std::max(1, c->num_cus) * per_cu;
+    }
+    int64_t max_grid = (a.count + kBlock - 1) / kBlock;
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, std::max<int64_t>(max_grid, 1)));
+    HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, 64, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    char err[256] = {0};
+    if (launch_transport(a, grid, c->stream, err, sizeof err)) return fail(c, LP_EDEVICE, "%s", err);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.kernel_ms = ms;
+    c->stats.photons = a.count;
+    c->stats.grid_blocks = grid;
+    c->stats.bvh_nodes = (int64_t)c->bvh.nodes.size();
+    c->stats.bvh_depth = c->bvh.depth;
+    return LP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lp_create(lp_context** out, int32_t device)
+{
+    if (!out) return LP_EINVAL;
+    *out = nullptr;
+    lp_context* c = new (std::nothrow) lp_context();
+    if (!c) return LP_ENOMEM;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0 || device < 0 || device >= n) {
+        delete c;
+        return LP_EDEVICE;
+    }
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return LP_EDEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    *out = c;
+    return LP_OK;
+}
+
+void lp_destroy(lp_context* c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety,
+                      &c->d_vstart, &c->d_vocc, &c->d_counter, &c->d_exit};
+    for (DevBuf* b : bufs) dev_free(*b);
+    for (auto& b : c->d_tally) dev_free(b);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* lp_last_error(const lp_context* c) { return c ? c->err.c_str() : "null context"; }
+
+int lp_set_mesh(lp_context* c, const float* tri, const float* nrm, const int32_t* med, int64_t ntri)
+{
+    if (!c) return LP_EINVAL;
+    if (ntri < 0 || ntri >= (1 << 24) || (ntri > 0 && (!tri || !nrm || !med)))
+        return fail(c, LP_EINVAL, "lp_set_mesh: bad arguments (ntri=%lld)", (long long)ntri);
+    c->tri.assign(tri, tri + 9 * ntri);
+    c->nrm.assign(nrm, nrm + 3 * ntri);
+    c->med.assign(med, med + 2 * ntri);
+    for (int64_t i = 0; i < 2 * ntri; ++i)
+        if (med[i] < 0 || med[i] > 0xffff) return fail(c, LP_EINVAL, "lp_set_mesh: tissue id out of range");
+    c->ntri = ntri;
+    c->mesh_dirty = true;
+    return LP_OK;
+}
+
+int lp_set_materials(lp_context* c, const float* nmuasg, int32_t ntissue, int32_t fetal_id)
+{
+    if (!c) return LP_EINVAL;
+    if (!nmuasg || ntissue < 2 || ntissue > kMaxTissues)
+        return fail(c, LP_EINVAL, "lp_set_materials: need 2..%d tissues", kMaxTissues);
+    c->nmuasg.assign(nmuasg, nmuasg + 4 * ntissue);
+    c->ntissue = ntissue;
+    c->fetal_id = fetal_id;
+    c->tallies_dirty = true;
+    return LP_OK;
+}
+
+int lp_set_source(lp_context* c, const float pstart[3], const float* vstart, const int32_t* vocc, int32_t nstart,
+                  const float poi_basis[12])
+{
+    if (!c) return LP_EINVAL;
+    if (!pstart || !vstart || !vocc || nstart < 1 || !poi_basis)
+        return fail(c, LP_EINVAL, "lp_set_source: bad arguments");
+    std::memcpy(c->pstart, pstart, 3 * sizeof(float));
+    c->vstart.assign(vstart, vstart + 3 * nstart);
+    c->vocc.assign(vocc, vocc + nstart);
+    std::memcpy(c->poi_basis, poi_basis, 12 * sizeof(float));
+    c->have_source = true;
+    c->src_dirty = true;
+    return LP_OK;
+}
+
+int lp_set_grid(lp_context* c, const float model_dims[3], const float det_grid[6])
+{
+    if (!c) return LP_EINVAL;
+    if (!model_dims || !det_grid) return fail(c, LP_EINVAL, "lp_set_grid: null");
+    for (int k = 0; k < 6; ++k)
+        if (!(det_grid[k] > 0)) return fail(c, LP_EINVAL, "lp_set_grid: detection grid must be > 0");
+    std::memcpy(c->model_dims, model_dims, 3 * sizeof(float));
+    std::memcpy(c->det_grid, det_grid, 6 * sizeof(float));
+    c->have_grid = true;
+    c->tallies_dirty = true;
+    return LP_OK;
+}
+
+int lp_set_params(lp_context* c, float chance, float w_th, float factor, uint64_t seed, int32_t rng_mode,
+                  int64_t max_steps)
+{
+    if (!c) return LP_EINVAL;
+    if (!(chance > 0 && chance <= 1) || !(w_th >= 0) || !(factor > 0) || (rng_mode != 0 && rng_mode != 1))
+        return fail(c, LP_EINVAL, "lp_set_params: bad arguments");
+    c->chance = chance;
+    c->w_th = w_th;
+    c->factor = factor;
+    c->seed = seed;
+    c->rng_mode = rng_mode;
+    c->max_steps = max_steps > 0 ? max_steps : 1000000;
+    return LP_OK;
+}
+
+int lp_set_voxel_grid(lp_context* c, const float origin[3], const float size[3], const int32_t n[3])
+{
+    if (!c) return LP_EINVAL;
+    if (!origin || !size || !n) return fail(c, LP_EINVAL, "lp_set_voxel_grid: null");
+    for (int k = 0; k < 3; ++k)
+        if (n[k] < 0 || (n[k] > 0 && !(size[k] > 0))) return fail(c, LP_EINVAL, "lp_set_voxel_grid: bad size");
+    std::memcpy(c->vox_origin, origin, sizeof c->vox_origin);
+    std::memcpy(c->vox_size, size, sizeof c->vox_size);
+    std::memcpy(c->vox_n, n, sizeof c->vox_n);
+    c->tallies_dirty = true;
+    return LP_OK;
+}
+
+int lp_set_radial(lp_context* c, float dr, int32_t nbins)
+{
+    if (!c) return LP_EINVAL;
+    if (nbins < 0 || (nbins > 0 && !(dr > 0))) return fail(c, LP_EINVAL, "lp_set_radial: bad arguments");
+    c->rad_dr = dr;
+    c->rad_n = nbins;
+    c->tallies_dirty = true;
+    return LP_OK;
+}
+
+int lp_set_detectors(lp_context* c, const float* rlo, const float* rhi, int32_t ndet)
+{
+    if (!c) return LP_EINVAL;
+    if (ndet < 0 || ndet > kMaxDetectors || (ndet > 0 && (!rlo || !rhi)))
+        return fail(c, LP_EINVAL, "lp_set_detectors: 0..%d detectors", kMaxDetectors);
+    c->det_rlo.assign(rlo, rlo + ndet);
+    c->det_rhi.assign(rhi, rhi + ndet);
+    c->tallies_dirty = true;
+    return LP_OK;
+}
+
+int lp_set_tallies(lp_context* c, uint32_t mask, int32_t record_exit)
+{
+    if (!c) return LP_EINVAL;
+    c->tally_mask = mask;
+    c->record_exit = record_exit;
+    return LP_OK;
+}
+
+int lp_set_launch(lp_context* c, int32_t grid_blocks, int32_t lds_nodes, int32_t safety_cells)
+{
+    if (!c) return LP_EINVAL;
+    if (lds_nodes > 2048) return fail(c, LP_EINVAL, "lp_set_launch: at most 2048 LDS nodes");
+    c->grid_override = grid_blocks;
+    c->lds_override = lds_nodes;
+    if (safety_cells != c->safety_cells) c->mesh_dirty = true;
+    c->safety_cells = safety_cells;
+    return LP_OK;
+}
+
+int lp_get_layout(lp_context* c, lp_layout* out)
+{
+    if (!c || !out) return LP_EINVAL;
+    compute_layout(c);
+    *out = c->L;
+    return LP_OK;
+}
+
+int lp_reset_tallies(lp_context* c)
+{
+    if (!c) return LP_EINVAL;
+    c->tallies_dirty = true;
+    return LP_OK;
+}
+
+int lp_run(lp_context* c, int64_t first, int64_t count)
+{
+    if (!c) return LP_EINVAL;
+    if (first < 0 || count < 0) return fail(c, LP_EINVAL, "lp_run: negative range");
+    int rc = prepare(c);
+    if (rc) return rc;
+    if (count == 0) return LP_OK;
+    KArgs a;
+    fill_args(c, a);
+    a.first = first;
+    a.count = count;
+    if (c->record_exit) {
+        if ((rc = dev_alloc(c, c->d_exit, (size_t)count * 8 * sizeof(float)))) return rc;
+        a.exit_rec = (float*)c->d_exit.p;
+        c->exit_count = count;
+    } else {
+        c->exit_count = 0;
+    }
+    return launch(c, a);
+}
+
+int lp_get_tally(lp_context* c, int32_t which, int64_t* out, int64_t n)
+{
+    if (!c) return LP_EINVAL;
+    if (which < 0 || which >= LP_NTALLY || (!out && n > 0)) return fail(c, LP_EINVAL, "lp_get_tally: bad tally");
+    compute_layout(c);
+    if (c->tallies_dirty || !c->d_tally[which].p) {
+        std::memset(out, 0, (size_t)n * sizeof(int64_t));
+        return LP_OK;
+    }
+    if (n > c->L.n[which]) return fail(c, LP_EINVAL, "lp_get_tally: %lld > %lld elements", (long long)n,
+                                       (long long)c->L.n[which]);
+    HIPCHK(c, hipMemcpyAsync(out, c->d_tally[which].p, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LP_OK;
+}
+
+int lp_get_exit_records(lp_context* c, float* out, int64_t n)
+{
+    if (!c || (!out && n > 0)) return LP_EINVAL;
+    if (n > c->exit_count) return fail(c, LP_ESTATE, "lp_get_exit_records: only %lld records", (long long)c->exit_count);
+    HIPCHK(c, hipMemcpyAsync(out, c->d_exit.p, (size_t)n * 8 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LP_OK;
+}
+
+int lp_trace(lp_context* c, int64_t first, int32_t count, int32_t max_rec, float* out, int32_t* nrec)
+{
+    if (!c) return LP_EINVAL;
+    if (first < 0 || count <= 0 || max_rec <= 0 || !out || !nrec) return fail(c, LP_EINVAL, "lp_trace: bad arguments");
+    int rc = prepare(c);
+    if (rc) return rc;
+    DevBuf tr, tn;
+    size_t tbytes = (size_t)count * max_rec * 12 * sizeof(float);
+    if ((rc = dev_alloc(c, tr, tbytes)) || (rc = dev_alloc(c, tn, (size_t)count * sizeof(int32_t)))) {
+        dev_free(tr);
+        dev_free(tn);
+        return rc;
+    }
+    KArgs a;
+    fill_args(c, a);
+    a.first = first;
+    a.count = count;
+    a.trace = (float*)tr.p;
+    a.trace_n = (int32_t*)tn.p;
+    a.trace_max = max_rec;
+    if (c->record_exit) {
+        if ((rc = dev_alloc(c, c->d_exit, (size_t)count * 8 * sizeof(float)))) return rc;
+        a.exit_rec = (float*)c->d_exit.p;
+        c->exit_count = count;
+    }
+    rc = launch(c, a);
+    if (!rc) {
+        hipError_t e1 = hipMemcpy(out, tr.p, tbytes, hipMemcpyDeviceToHost);
+        hipError_t e2 = hipMemcpy(nrec, tn.p, (size_t)count * sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (e1 != hipSuccess || e2 != hipSuccess) rc = fail(c, LP_EDEVICE, "lp_trace: copy failed");
+    }
+    dev_free(tr);
+    dev_free(tn);
+    return rc;
+}
+
+int lp_get_stats(lp_context* c, lp_stats* out)
+{
+    if (!c || !out) return LP_EINVAL;
+    int64_t bytes = 0;
+    DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety,
+                      &c->d_vstart, &c->d_vocc, &c->d_counter, &c->d_exit};
+    for (DevBuf* b : bufs) bytes += (int64_t)b->bytes;
+    for (auto& b : c->d_tally) bytes += (int64_t)b.bytes;
+    c->stats.device_bytes = bytes;
+    c->stats.bvh_nodes = (int64_t)c->bvh.nodes.size();
+    c->stats.bvh_depth = c->bvh.depth;
+    *out = c->stats;
+    return LP_OK;
+}
+
+int lp_debug_math(lp_context* c, int32_t op, const float* xy, float* out, int64_t n)
+{
+    if (!c || !xy || !out || n < 0) return LP_EINVAL;
+    if (n == 0) return LP_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf din, dout;
+    int rc;
+    if ((rc = dev_alloc(c, din, (size_t)n * 2 * sizeof(float))) || (rc = dev_alloc(c, dout, (size_t)n * sizeof(float)))) {
+        dev_free(din);
+        dev_free(dout);
+        return rc;
+    }
+    rc = LP_OK;
+    if (hipMemcpy(din.p, xy, (size_t)n * 2 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_debug_math(op, (const float*)din.p, (float*)dout.p, n, nullptr) != 0 ||
+        hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out, dout.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(c, LP_EDEVICE, "lp_debug_math failed");
+    dev_free(din);
+    dev_free(dout);
+    return rc;
+}
+
+}  // extern "C"

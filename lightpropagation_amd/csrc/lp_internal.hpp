@@ -1,0 +1,118 @@
+// lp_internal.hpp -- data layout shared by the host runtime and the kernels.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+namespace lp {
+
+// Threaded (stackless) BVH node, 32 bytes.  Traversal starts at node 0 and
+// follows `hit` when the ray segment meets the (padded) box and `miss`
+// otherwise; miss == -1 ends the walk.  For a leaf, hit < 0 encodes the
+// triangle range: first = hit & 0xffffff, count = (hit >> 24) & 0x7f, and
+// after the leaf's triangles are tested the walk continues at `miss`.
+struct BvhNode {
+    float lo[3];
+    int32_t hit;
+    float hi[3];
+    int32_t miss;
+};
+static_assert(sizeof(BvhNode) == 32, "node layout");
+constexpr int kMaxLeaf = 127;
+
+// Leaf-ordered triangle record, 48 bytes: (M, original id), (E = K - M,
+// packed inner | outer << 16), (F = L - M, 0).  E and F are the exact float
+// differences FindTriangleInPath forms (MCLightPropagation.cl:1058-1060).
+struct TriRec {
+    float m[3];
+    int32_t id;
+    float e[3];
+    int32_t med;
+    float f[3];
+    int32_t pad;
+};
+static_assert(sizeof(TriRec) == 48, "triangle layout");
+
+struct Bvh {
+    std::vector<BvhNode> nodes;  // breadth-first order (top levels first), root at 0
+    std::vector<TriRec> tris;    // leaf order
+    int32_t depth = 0;
+    float pad = 0.0f;
+};
+
+// Builds a binned-SAH BVH over ntri triangles (9 floats each).  Child boxes
+// are widened by `pad` (absolute) so that culling stays conservative with
+// respect to the rounding of the exact triangle test.
+Bvh build_bvh(const float* tri, const int32_t* med, int64_t ntri, int max_leaf = 4);
+
+// Conservative lower bound of the distance from any point of a grid cell to
+// the mesh, used to skip the triangle search for steps that cannot reach a
+// surface.  Cells hold a float distance; 0 where a surface is close.
+struct SafetyGrid {
+    float origin[3] = {0, 0, 0};
+    float inv_cell[3] = {0, 0, 0};
+    int32_t n[3] = {0, 0, 0};
+    std::vector<float> dist;
+};
+SafetyGrid build_safety_grid(const Bvh& bvh, const float* tri, int64_t ntri, int target_cells_per_axis);
+
+constexpr int kMaxTissues = 64;
+constexpr int kMaxDetectors = 64;
+constexpr int kBlock = 256;
+
+// Kernel arguments (passed by value).
+struct KArgs {
+    // scene
+    const TriRec* tris;
+    const BvhNode* nodes;
+    int32_t nnodes;
+    int32_t nlds;  // nodes [0, nlds) are staged in LDS
+    const float* nrm;     // 3 per original triangle
+    const int32_t* med;   // 2 per original triangle
+    int32_t ntri;
+    const float* safety;  // optional safety grid
+    float sg_origin[3], sg_inv[3];
+    int32_t sg_n[3];
+    float nmuasg[4 * kMaxTissues];
+    int32_t ntissue, fetal_id;
+    float pstart[3];
+    const float* vstart;
+    const int32_t* vocc;
+    int32_t nstart;
+    float poi[3], ex[3], ey[3], ez[3];  // eX, eY, eZ already normalised
+    float ez_raw[3];                    // eZ as given (sign of the near-axis override)
+    float det_grid[6];
+    float rsp, dist_min;
+    int32_t dimx, dimy, dimz, dimR, dimAlpha, sft_nx, sft_ny, arz_nr, arz_nz;
+    float factor, chance, w_th;
+    uint64_t seed;
+    int64_t max_steps;
+    int32_t rng_mode;
+    // tallies (null = off)
+    unsigned long long *sft, *rtd, *arz, *phirz, *region, *vox, *rad, *det, *misc;
+    float vox_origin[3], vox_size[3];
+    int32_t vox_n[3];
+    float rad_dr;
+    int32_t rad_n;
+    int32_t ndet;
+    float det_rlo[kMaxDetectors], det_rhi[kMaxDetectors];
+    // per-photon outputs (null = off)
+    float* exit_rec;
+    float* trace;
+    int32_t* trace_n;
+    int32_t trace_max;
+    // work
+    int64_t first, count;
+    unsigned long long* counter;
+};
+
+enum MiscSlot {
+    kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
+    kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscN = 16
+};
+
+// Launchers (defined in lp_kernel.hip).
+int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen);
+int launch_debug_math(int op, const float* in, float* out, int64_t n, void* stream);
+
+}  // namespace lp

@@ -1,0 +1,711 @@
+// lp_kernel.hip -- persistent-thread photon transport on triangle meshes
+// for gfx950 (MI355X).
+//
+// One lane carries one photon packet through the loop body of the reference
+// kernel (src/Simulation/MCLightPropagation.cl:1375-1575): nearest face
+// ahead, step length, boundary branch (Fresnel reflect/refract) or
+// interaction branch (absorb, Henyey-Greenstein spin), Russian roulette.
+// The grid is persistent: each wavefront refills lanes whose photon died
+// from a global photon counter with one atomic per refill (wave-level
+// compaction), so lanes stay busy until the photon pool is drained.
+//
+// Differences of design from the reference (results are the same as the
+// oracle's restatement of the reference algorithm, bit for bit):
+//  * the nearest-face query runs after the step length is drawn and is
+//    bounded by it (it draws no random numbers, so the stream is unchanged);
+//    a step shorter than the conservative distance to the mesh skips it;
+//  * faces are found through a threaded BVH whose top levels sit in LDS;
+//  * per-photon counter-based (Philox) or per-photon ran3 streams replace
+//    the work-group-shared, lock-ordered ran3 state (.cl:1245-1259);
+//  * all tallies are 64-bit fixed-point integer atomics.
+#include <hip/hip_runtime.h>
+
+#include "lp_devmath.hpp"
+#include "lp_internal.hpp"
+
+namespace lp {
+
+
+struct Ran3Lds {
+    // ran3 state of each lane, lane-minor so neighbouring lanes use
+    // neighbouring banks: s[i * kBlock + tid]
+    int32_t s[56 * kBlock];
+};
+
+// ---------------------------------------------------------------------------
+// random numbers
+// ---------------------------------------------------------------------------
+struct PhiloxState {
+    uint32_t k0, k1, id0, id1, block;
+    uint32_t buf[4];
+    int used;
+    __device__ void init(uint64_t seed, uint64_t photon)
+    {
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32);
+        id0 = (uint32_t)photon;
+        id1 = (uint32_t)(photon >> 32);
+        block = 0;
+        used = 4;
+    }
+    __device__ __forceinline__ float next()
+    {
+        if (used == 4) {
+            uint32_t c[4] = {block, id0, id1, 0u};
+            philox10(c, k0, k1);
+            buf[0] = c[0];
+            buf[1] = c[1];
+            buf[2] = c[2];
+            buf[3] = c[3];
+            block++;
+            used = 0;
+        }
+        uint32_t u;
+        switch (used) {  // keeps buf[] in registers
+        case 0: u = buf[0]; break;
+        case 1: u = buf[1]; break;
+        case 2: u = buf[2]; break;
+        default: u = buf[3]; break;
+        }
+        used++;
+        return (float)(u >> 8) * 5.96046448e-08f;
+    }
+};
+
+// PSNR::InitRand3 (src/PSNR/PSNR.cpp:48-80) in int32 (all values < 2^31),
+// its first draw, then the float conversion the host does (GetRan3Array).
+// The state is kept as 32-bit words; float words are bit casts.
+__device__ void ran3_init_lds(int32_t* m, int tid, uint64_t seed, uint64_t photon, int& inext, int& inextp)
+{
+    const int32_t MBIG = 1000000000, MSEED = 161803398;
+    int32_t sd = (int32_t)((seed + photon) % (uint64_t)MSEED);
+    int32_t mj = (MSEED - sd) % MBIG;
+    m[0 * kBlock + tid] = 0;
+    m[55 * kBlock + tid] = mj;
+    int32_t mk = 1;
+    for (int i = 1; i <= 54; i++) {
+        int ii = (21 * i) % 55;
+        m[ii * kBlock + tid] = mk;
+        mk = mj - mk;
+        if (mk < 0) mk += MBIG;
+        mj = m[ii * kBlock + tid];
+    }
+    for (int k = 1; k <= 4; k++)
+        for (int i = 1; i <= 55; i++) {
+            int32_t v = m[i * kBlock + tid] - m[(1 + (i + 30) % 55) * kBlock + tid];
+            if (v < 0) v += MBIG;
+            m[i * kBlock + tid] = v;
+        }
+    inext = 1;
+    inextp = 32;
+    int32_t d = m[1 * kBlock + tid] - m[32 * kBlock + tid];
+    if (d < 0) d += MBIG;
+    m[1 * kBlock + tid] = d;
+    for (int i = 0; i < 56; ++i) m[i * kBlock + tid] = __float_as_int((float)m[i * kBlock + tid]);
+}
+
+// DoRand3 (.cl:1204-1243), one stream per lane.
+__device__ __forceinline__ float ran3f_next(int32_t* m, int tid, int& inext, int& inextp)
+{
+    int i = inext + 1, j = inextp + 1;
+    if (i == 56) i = 1;
+    if (j == 56) j = 1;
+    float r = __int_as_float(m[i * kBlock + tid]) - __int_as_float(m[j * kBlock + tid]);
+    if (r < 0.0f) r = r + 1000000000.0f;
+    m[i * kBlock + tid] = __float_as_int(r);
+    inext = i;
+    inextp = j;
+    return r / 1000000000.0f;
+}
+
+template <int RNG>
+struct Rng {
+    PhiloxState ph;
+    int inext, inextp;
+    int32_t* s;
+    int tid;
+    int64_t ndraw;
+    __device__ void init(uint64_t seed, uint64_t photon)
+    {
+        ndraw = 0;
+        if (RNG == 0)
+            ph.init(seed, photon);
+        else
+            ran3_init_lds(s, tid, seed, photon, inext, inextp);
+    }
+    __device__ __forceinline__ float next()
+    {
+        ndraw++;
+        if (RNG == 0) return ph.next();
+        return ran3f_next(s, tid, inext, inextp);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// nearest face ahead
+// ---------------------------------------------------------------------------
+struct Hit {
+    float t;
+    int32_t id;
+    int32_t med;
+};
+
+// Nearest face along p + t v with 0 < t < 10000 and t <= tmax, lowest
+// original index among equal t, ref excluded: the argmin FindTriangleInPath
+// (.cl:1037-1093) computes, restricted to t <= tmax.
+__device__ __forceinline__ void find_face(const KArgs& a, const float4* lnodes, f3 p, f3 v, int32_t ref,
+                                          float tmax, Hit& h, uint32_t& visits, uint32_t& tests)
+{
+    h.t = tmax;
+    h.id = -1;
+    h.med = 0;
+    if (a.nnodes == 0) return;
+    const float ix = (v.x != 0.0f) ? 1.0f / v.x : 3.0e38f;
+    const float iy = (v.y != 0.0f) ? 1.0f / v.y : 3.0e38f;
+    const float iz = (v.z != 0.0f) ? 1.0f / v.z : 3.0e38f;
+    const float4* gn = reinterpret_cast<const float4*>(a.nodes);
+    int32_t n = 0;
+    while (n >= 0) {
+        float4 lo, hi;
+        if (n < a.nlds) {
+            lo = lnodes[2 * n];
+            hi = lnodes[2 * n + 1];
+        } else {
+            lo = gn[2 * n];
+            hi = gn[2 * n + 1];
+        }
+        visits++;
+        float tx0 = (lo.x - p.x) * ix, tx1 = (hi.x - p.x) * ix;
+        float ty0 = (lo.y - p.y) * iy, ty1 = (hi.y - p.y) * iy;
+        float tz0 = (lo.z - p.z) * iz, tz1 = (hi.z - p.z) * iz;
+        float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), h.t));
+        int32_t hit = __float_as_int(lo.w), miss = __float_as_int(hi.w);
+        if (tn <= tf) {
+            if (hit >= 0) {
+                n = hit;
+                continue;
+            }
+            int32_t first = hit & 0xffffff, cnt = (hit >> 24) & 0x7f;
+            const float4* tr = reinterpret_cast<const float4*>(a.tris) + 3 * (int64_t)first;
+            for (int32_t j = 0; j < cnt; ++j) {
+                float4 A = tr[3 * j], B = tr[3 * j + 1], C = tr[3 * j + 2];
+                int32_t id = __float_as_int(A.w);
+                tests++;
+                if (id == ref) continue;
+                f3 E = mk(B.x, B.y, B.z), F = mk(C.x, C.y, C.z);
+                f3 G = sub(p, mk(A.x, A.y, A.z));
+                f3 GxE = cross(G, E), vxF = cross(v, F);
+                float w = dot(vxF, E);
+                if (w == 0.0f) continue;  // t = +-inf or NaN: never accepted
+                float tx = dot(GxE, F);
+                bool wp = w > 0.0f;
+                if ((tx > 0.0f) != wp) continue;  // t = tx/w > 0 needs equal signs
+                float ty = dot(vxF, G), tz = dot(GxE, v);
+                float tiny = __builtin_fabsf(w) * 1e-30f;
+                if (!(ty == 0.0f || (ty > 0.0f) == wp || __builtin_fabsf(ty) < tiny)) continue;
+                if (!(tz == 0.0f || (tz > 0.0f) == wp || __builtin_fabsf(tz) < tiny)) continue;
+                float t = tx / w, k = ty / w, l = tz / w;
+                if (t > 0.0f && k >= 0.0f && l >= 0.0f && (k + l) < 1.0f && t < 10000.0f &&
+                    (t < h.t || (t == h.t && (h.id < 0 || id < h.id)))) {
+                    h.t = t;
+                    h.id = id;
+                    h.med = __float_as_int(B.w);
+                }
+            }
+        }
+        n = miss;
+    }
+}
+
+__device__ __forceinline__ float safety_at(const KArgs& a, f3 p)
+{
+    if (!a.safety) return 0.0f;
+    float fx = (p.x - a.sg_origin[0]) * a.sg_inv[0];
+    float fy = (p.y - a.sg_origin[1]) * a.sg_inv[1];
+    float fz = (p.z - a.sg_origin[2]) * a.sg_inv[2];
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f)) return 0.0f;
+    int ix = (int)fminf(fx, 1e9f), iy = (int)fminf(fy, 1e9f), iz = (int)fminf(fz, 1e9f);
+    if (ix >= a.sg_n[0] || iy >= a.sg_n[1] || iz >= a.sg_n[2]) return 0.0f;
+    return a.safety[((int64_t)iz * a.sg_n[1] + iy) * a.sg_n[0] + ix];
+}
+
+__device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v)
+{
+    if (v) atomicAdd(p, v);
+}
+
+// ---------------------------------------------------------------------------
+// the transport kernel
+// ---------------------------------------------------------------------------
+struct Photon {
+    f3 p, v, nv, sft;
+    float w, unfin, absorbed;
+    int32_t ref, curr;
+    int32_t fetal, roulette;
+    int64_t steps;
+    int64_t k;  // index within this run
+};
+
+template <int RNG>
+__global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
+{
+    extern __shared__ float4 smem[];
+    float4* lnodes = smem;                         // 2 * nlds
+    float4* lmat = smem + 2 * a.nlds;              // ntissue
+    int32_t* lran3 = reinterpret_cast<int32_t*>(lmat + kMaxTissues);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    {
+        const float4* gn = reinterpret_cast<const float4*>(a.nodes);
+        for (int i = tid; i < 2 * a.nlds; i += kBlock) lnodes[i] = gn[i];
+        for (int i = tid; i < a.ntissue; i += kBlock)
+            lmat[i] = make_float4(a.nmuasg[4 * i], a.nmuasg[4 * i + 1], a.nmuasg[4 * i + 2], a.nmuasg[4 * i + 3]);
+    }
+    __syncthreads();
+
+    const f3 POI = mk(a.poi[0], a.poi[1], a.poi[2]);
+    const f3 eX = mk(a.ex[0], a.ex[1], a.ex[2]);
+    const f3 eY = mk(a.ey[0], a.ey[1], a.ey[2]);
+    const f3 eZ = mk(a.ez[0], a.ez[1], a.ez[2]);
+    const f3 eZr = mk(a.ez_raw[0], a.ez_raw[1], a.ez_raw[2]);
+
+    Rng<RNG> rng;
+    rng.s = lran3;
+    rng.tid = tid;
+    Photon ph;
+    bool alive = false;
+    bool exhausted = false;  // wave-uniform
+    uint64_t c_steps = 0, c_int = 0, c_bnd = 0, c_esc = 0, c_cap = 0, c_phot = 0, c_skip = 0;
+    uint64_t c_arz_out = 0, c_vox_out = 0;
+    uint32_t c_visits = 0, c_tests = 0;
+    uint64_t c_visits64 = 0, c_tests64 = 0;
+    unsigned long long reg_acc = 0;  // fixed-point absorption in ph.curr not yet flushed
+
+    for (;;) {
+        // ---- refill lanes whose photon has finished (one atomic per wave) ----
+        if (!exhausted) {
+            unsigned long long need = __ballot(!alive);
+            if (need) {
+                int cnt = __popcll(need);
+                int leader = __ffsll((long long)need) - 1;
+                unsigned long long got = 0;
+                if (lane == leader) got = atomicAdd(a.counter, (unsigned long long)cnt);
+                uint32_t blo = __shfl((uint32_t)got, leader), bhi = __shfl((uint32_t)(got >> 32), leader);
+                unsigned long long base = ((unsigned long long)bhi << 32) | blo;
+                int rank = __popcll(need & ((1ull << lane) - 1ull));
+                if (base + (unsigned long long)cnt >= (unsigned long long)a.count) exhausted = true;
+                if (!alive) {
+                    unsigned long long k = base + (unsigned long long)rank;
+                    if (k < (unsigned long long)a.count) {
+                        int64_t id = a.first + (int64_t)k;
+                        rng.init(a.seed, (uint64_t)id);
+                        // GetvStart (.cl:423-467): A == 1 -> vStart[0]; A > 1 by cumulative counts
+                        int dsel = 0;
+                        if (a.nstart > 1) {
+                            int64_t sum = 0;
+                            for (int i = 0; i < a.nstart; ++i) {
+                                sum += a.vocc[i];
+                                dsel = i;
+                                if (id < sum) break;
+                            }
+                        }
+                        // PhotonClass::Init (.cl:344-389)
+                        ph.v = mk(a.vstart[3 * dsel], a.vstart[3 * dsel + 1], a.vstart[3 * dsel + 2]);
+                        ph.p = mk(a.pstart[0], a.pstart[1], a.pstart[2]);
+                        ph.w = 1.0f - a.rsp;
+                        ph.sft = mk(0.0f, 0.0f, 0.0f);
+                        ph.unfin = 0.0f;
+                        ph.absorbed = 0.0f;
+                        ph.fetal = 0;
+                        ph.ref = -1;
+                        ph.curr = 0;
+                        ph.nv = mk(0.0f, 0.0f, -1.0f);
+                        ph.roulette = 1;
+                        ph.steps = 0;
+                        ph.k = (int64_t)k;
+                        reg_acc = 0;
+                        alive = true;
+                        c_phot++;
+                    }
+                }
+            }
+        }
+        if (!__any(alive)) break;
+        if (!alive) continue;
+
+        // ---- one iteration of the loop body (.cl:1375-1575) ----
+        const float4 cm = lmat[ph.curr];  // (n, mua, mus, g)
+        float stepsize;
+        bool ballistic = (cm.y == 0.0f && cm.z == 0.0f);
+        if (!ballistic) {
+            if (ph.unfin == 0.0f) {
+                float x;
+                do {
+                    x = rng.next();
+                } while (x <= 0.0f);
+                stepsize = -logf_lp(x) * a.factor / (cm.y + cm.z);
+            } else {
+                stepsize = ph.unfin / (cm.y + cm.z);
+                ph.unfin = 0.0f;
+            }
+        }
+        Hit h;
+        if (!ballistic && stepsize < safety_at(a, ph.p)) {
+            h.id = -1;
+            c_skip++;
+        } else {
+            find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, c_visits, c_tests);
+        }
+        float stb = 10000.0f;
+        bool boundary;
+        if (h.id >= 0) {
+            stb = h.t;
+            boundary = true;
+        } else {
+            boundary = ballistic || stepsize >= 10000.0f;  // no face ahead at all
+        }
+        if (ballistic) stepsize = stb;
+        int kind;
+        if (boundary) {
+            ph.p = add(ph.p, mul(stb, ph.v));  // MoveStep
+            ph.ref = -1;
+            if (h.id < 0) {  // nothing ahead: the photon has left the mesh
+                kind = 3;
+                ph.w = 0.0f;
+                c_esc++;
+            } else {
+                float x = rng.next();
+                int32_t in_id = h.med & 0xffff, out_id = (h.med >> 16) & 0xffff;
+                ph.nv = normalize(mk(a.nrm[3 * h.id], a.nrm[3 * h.id + 1], a.nrm[3 * h.id + 2]));
+                int32_t nx_id = (ph.curr == out_id) ? in_id : out_id;
+                const float4 nm = lmat[nx_id];
+                if (__builtin_fabsf(__builtin_sqrtf(ph.p.x * ph.p.x + ph.p.y * ph.p.y)) < 0.01f)
+                    ph.nv = mul(sign_cl(dot(ph.nv, eZr)), eZ);
+                // TransmitOrReflect (.cl:921-1018)
+                float cos_in = dot(ph.nv, ph.v);
+                float abs_cos = __builtin_fabsf(cos_in);
+                float n1 = cm.x, n2 = nm.x;
+                float cos_crit = 0.0f;
+                if (n1 > n2) {
+                    float q = n2 / n1;
+                    cos_crit = __builtin_sqrtf(1.0f - q * q);
+                }
+                float r;
+                if (abs_cos <= cos_crit) {
+                    r = 1.0f;
+                } else if (n1 == n2) {  // GetFresnel (.cl:857-907)
+                    r = 0.0f;
+                } else if (abs_cos == 1.0f) {
+                    r = (n1 - n2) / (n1 + n2);
+                    r *= r;
+                } else if (cos_in == 0.0f) {
+                    r = 1.0f;
+                } else {
+                    float sin_in = __builtin_sqrtf(1 - cos_in * cos_in);
+                    float sin_out = n1 * sin_in / n2;
+                    if (sin_out >= 1.0f) {
+                        r = 1.0f;
+                    } else {
+                        float cos_out = __builtin_sqrtf(1 - sin_out * sin_out);
+                        float cap = cos_in * cos_out - sin_in * sin_out;
+                        float cam = cos_in * cos_out + sin_in * sin_out;
+                        float sap = sin_in * cos_out + cos_in * sin_out;
+                        float sam = sin_in * cos_out - cos_in * sin_out;
+                        r = 0.5f * (sam * sam) * ((cam * cam) + (cap * cap)) / ((sap * sap) * (cam * cam));
+                    }
+                }
+                f3 ey = normalize(cross(ph.nv, cross(ph.nv, ph.v)));
+                int32_t new_id;
+                float m_a, m_s;
+                if (x > r) {  // Transmit (.cl:506-529)
+                    float sin_t = __builtin_sqrtf(1 - cos_in * cos_in) * n1 / n2;
+                    float cos_t = -__builtin_sqrtf(1 - sin_t * sin_t);
+                    if (cos_in > 0.0f) cos_t = __builtin_fabsf(cos_t);
+                    ph.v = normalize(sub(mul(cos_t, ph.nv), mul(sin_t, ey)));
+                    new_id = (cos_in > 0) ? out_id : in_id;
+                    m_a = nm.y;
+                    m_s = nm.z;
+                    kind = 1;
+                } else {  // Reflect (.cl:532-549)
+                    float sin_r = __builtin_sqrtf(1 - cos_in * cos_in);
+                    ph.v = neg(normalize(add(mul(cos_in, ph.nv), mul(sin_r, ey))));
+                    new_id = ph.curr;
+                    m_a = cm.y;
+                    m_s = cm.z;
+                    kind = 2;
+                }
+                if (m_a == 0 || m_s == 0) {  // leaves the tissue: record and kill
+                    ph.sft.y = 0;
+                    float dist = __builtin_fabsf(dot(eZ, sub(ph.p, POI)));
+                    float wr = ph.w * (1 - a.rsp);
+                    if (dist > a.dist_min) {
+                        ph.sft.z = wr;
+                    } else {
+                        if (ph.fetal) ph.sft.y = wr;
+                        ph.sft.x = wr;
+                    }
+                    ph.w = 0.0f;
+                }
+                if (new_id != ph.curr) {
+                    if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
+                    reg_acc = 0;
+                }
+                ph.curr = new_id;
+                ph.ref = h.id;
+                if (ph.curr == a.fetal_id) ph.fetal = 1;
+                ph.unfin = (stepsize - stb) * (cm.y + cm.z);
+            }
+            c_bnd++;
+        } else {
+            // interaction branch (.cl:1513-1537)
+            ph.p = add(ph.p, mul(stepsize, ph.v));
+            ph.ref = -1;
+            if (cm.y > 0.0f && cm.z > 0.0f) {  // Absorb (.cl:841-847)
+                ph.absorbed = ph.w * (cm.y / (cm.y + cm.z));
+                ph.w -= ph.absorbed;
+            }
+            float xt = rng.next();
+            float xp = rng.next();
+            {  // VScatter (.cl:597-655)
+                const f3 v = ph.v;
+                const float g = cm.w;
+                f3 e_rot = mk(0.0f, 0.0f, 0.0f);
+                if (v.z != 0.0f) {
+                    float z = -(v.x + v.y) / v.z;
+                    e_rot = mk(1.0f, 1.0f, z);
+                } else if (v.y != 0.0f) {
+                    float y = -(v.x) / v.y;
+                    e_rot = mk(1.0f, y, 0.0f);
+                } else if (v.x != 0.0f) {
+                    e_rot = mk(0.0f, 1.0f, 0.0f);
+                }
+                e_rot = normalize(e_rot);
+                float cos_t;
+                float g1 = (1 - g * g) / (1 - g + 2 * g * xt);
+                if (g != 0.0f) {
+                    cos_t = (1 + g * g - g1 * g1) / (2 * g);
+                    if (cos_t < -1) cos_t = -1;
+                    else if (cos_t > 1) cos_t = 1;
+                } else {
+                    cos_t = 2 * xt - 1;
+                }
+                float sin_t = __builtin_sqrtf(1.0f - cos_t * cos_t);
+                float psi = 6.28318548f * xp;
+                float cos_p = cosf_lp(psi);
+                float sin_p = __builtin_sqrtf(1.0f - cos_p * cos_p);
+                if (psi > 3.14159274f) sin_p *= -1;
+                // Rotate (.cl:697-708) twice
+                float d1 = dot(e_rot, v);
+                f3 c1 = cross(e_rot, v);
+                f3 c1n = cross(c1, e_rot);
+                f3 rt = normalize(add(add(mul(d1, e_rot), mul(cos_t, c1n)), mul(sin_t, c1)));
+                float d2 = dot(v, rt);
+                f3 c2 = cross(v, rt);
+                f3 c2n = cross(c2, v);
+                ph.v = normalize(add(add(mul(d2, v), mul(cos_p, c2n)), mul(sin_p, c2)));
+            }
+            // absorption tallies at the new position (replaces RecordAbsorbtionRZ, .cl:254-271)
+            const f3 d = sub(ph.p, POI);
+            const f3 c = mk(dot(eX, d), dot(eY, d), dot(eZ, d));
+            unsigned long long q = fx32(ph.absorbed);
+            reg_acc += q;
+            if (a.arz) {
+                float rho = __builtin_sqrtf(c.x * c.x + c.y * c.y) * sign_cl(c.x) / a.det_grid[3];
+                int32_t ir = cvt_i32f(rho);
+                int32_t iz = cvt_i32f(c.z / a.det_grid[2]);
+                if (ir >= -a.dimR && ir <= a.dimR && iz >= 0 && iz <= a.dimz) {
+                    int64_t cell = (int64_t)iz * a.arz_nr + (ir + a.dimR);
+                    atomic_add_u64(&a.arz[cell], q);
+                    atomic_add_u64(&a.phirz[cell], fx24(ph.absorbed / (cm.y == 0.0f ? 1.0f : cm.y)));
+                } else {
+                    c_arz_out += q;
+                }
+            }
+            if (a.vox) {
+                float fx = (c.x - a.vox_origin[0]) / a.vox_size[0];
+                float fy = (c.y - a.vox_origin[1]) / a.vox_size[1];
+                float fz = (c.z - a.vox_origin[2]) / a.vox_size[2];
+                bool ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f;
+                int32_t ix = cvt_i32f(fx), iy = cvt_i32f(fy), iz = cvt_i32f(fz);
+                ok = ok && ix >= 0 && iy >= 0 && iz >= 0 && ix < a.vox_n[0] && iy < a.vox_n[1] && iz < a.vox_n[2];
+                if (ok)
+                    atomic_add_u64(&a.vox[((int64_t)iz * a.vox_n[1] + iy) * a.vox_n[0] + ix], q);
+                else
+                    c_vox_out += q;
+            }
+            c_int++;
+            kind = 0;
+        }
+        // Roulette (.cl:1020-1035, 1539-1545)
+        if (ph.w < a.w_th && ph.w > 0.0f) {
+            float x = rng.next();
+            if (x < a.chance) {
+                ph.w /= a.chance;
+                ph.roulette = 1;
+            } else {
+                ph.w = 0;
+                ph.roulette = 0;
+            }
+        }
+        ph.steps++;
+        c_steps++;
+        if (a.trace && ph.steps <= a.trace_max) {
+            float* t = a.trace + ((int64_t)ph.k * a.trace_max + (ph.steps - 1)) * 12;
+            t[0] = (float)kind;
+            t[1] = (float)(boundary ? h.id : -1);
+            t[2] = (float)ph.curr;
+            t[3] = ph.w;
+            t[4] = ph.p.x;
+            t[5] = ph.p.y;
+            t[6] = ph.p.z;
+            t[7] = ph.v.x;
+            t[8] = ph.v.y;
+            t[9] = ph.v.z;
+            t[10] = stepsize;
+            t[11] = (float)rng.ndraw;
+        }
+        if (a.max_steps > 0 && ph.steps >= a.max_steps && ph.w > 0.0f) {
+            c_cap++;
+            ph.w = 0.0f;
+        }
+        if (ph.w > 0.0f && (ph.w > a.w_th || ph.roulette)) continue;
+
+        // ---- epilogue (.cl:1586-1594) + host binning (cpp:1553-1630) ----
+        alive = false;
+        if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
+        reg_acc = 0;
+        float r8[8];
+        {
+            const f3 d = sub(ph.p, POI);
+            const f3 dn = sub(POI, ph.p);
+            r8[0] = dot(eX, d);
+            r8[1] = dot(eY, d);
+            r8[2] = dot(eZ, d);
+            r8[3] = ph.sft.x;
+            r8[4] = ph.sft.y;
+            r8[5] = ph.sft.z;
+            r8[6] = __builtin_fabsf(acosf_lp(dot(ph.v, ph.nv)));
+            r8[7] = __builtin_sqrtf((dn.x * dn.x + dn.y * dn.y) + dn.z * dn.z);
+        }
+        if (a.exit_rec) {
+            float4* o = reinterpret_cast<float4*>(a.exit_rec + 8 * ph.k);
+            o[0] = make_float4(r8[0], r8[1], r8[2], r8[3]);
+            o[1] = make_float4(r8[4], r8[5], r8[6], r8[7]);
+        }
+        if (a.trace_n) a.trace_n[ph.k] = (int32_t)(ph.steps < a.trace_max ? ph.steps : a.trace_max);
+        if (r8[3] != 0.0f || r8[4] != 0.0f || r8[5] != 0.0f) {
+            unsigned long long q0 = fx32(r8[3]), q1 = fx32(r8[4]), q2 = fx32(r8[5]);
+            if (a.sft) {
+                int32_t ex0 = -(a.dimx / 2), ex1 = a.dimx / 2, ey0 = -(a.dimy / 2), ey1 = a.dimy / 2;
+                int32_t x = cvt_i32f(r8[0] / a.det_grid[0]);
+                int32_t y = cvt_i32f(r8[1] / a.det_grid[1]);
+                y = (y < ey0) ? ey0 : y;
+                y = (y > ey1) ? ey1 : y;
+                if (x < ex0) x = ex0;
+                else if (x > ex1) x = ex1;
+                int64_t px = ((int64_t)(y - ey0) * a.sft_nx + (x - ex0)) * 3;
+                atomic_add_u64(&a.sft[px], q0);
+                atomic_add_u64(&a.sft[px + 1], q1);
+                atomic_add_u64(&a.sft[px + 2], q2);
+            }
+            if (a.rtd) {
+                int32_t ia = cvt_i32(((double)(r8[6] * 180.0f) / 3.14159265358979323846) / (double)a.det_grid[5]);
+                int32_t ir = cvt_i32f(r8[7] / a.det_grid[3]);
+                if (ir > a.dimR - 1) ir = a.dimR - 1;
+                else if (ir < 0) ir = 0;
+                if (ia > a.dimAlpha - 1) ia = a.dimAlpha - 1;
+                else if (ia < 0) ia = 0;
+                int64_t px = ((int64_t)ia * a.dimR + ir) * 3;
+                atomic_add_u64(&a.rtd[px], q0);
+                atomic_add_u64(&a.rtd[px + 1], q1);
+                atomic_add_u64(&a.rtd[px + 2], q2);
+            }
+            if (a.rad) {
+                int32_t ib = cvt_i32f(r8[7] / a.rad_dr);
+                if (ib >= 0 && ib < a.rad_n) {
+                    atomic_add_u64(&a.rad[3 * ib], q0);
+                    atomic_add_u64(&a.rad[3 * ib + 1], q1);
+                    atomic_add_u64(&a.rad[3 * ib + 2], q2);
+                }
+            }
+            if (a.det) {
+                for (int kd = 0; kd < a.ndet; ++kd) {
+                    if (r8[7] >= a.det_rlo[kd] && r8[7] < a.det_rhi[kd]) {
+                        atomic_add_u64(&a.det[2 * kd], q0);
+                        atomic_add_u64(&a.det[2 * kd + 1], q1);
+                    }
+                }
+            }
+        }
+        if (c_visits > 0x40000000u) { c_visits64 += c_visits; c_visits = 0; }
+        if (c_tests > 0x40000000u) { c_tests64 += c_tests; c_tests = 0; }
+    }
+    c_visits64 += c_visits;
+    c_tests64 += c_tests;
+    if (a.misc) {
+        uint64_t vals[kMiscN] = {c_phot, c_steps, c_int, c_bnd, c_esc, c_cap, c_arz_out, c_vox_out,
+                                 c_visits64, c_tests64, c_skip, 0, 0, 0, 0, 0};
+        for (int i = 0; i < 11; ++i) {
+            uint64_t v = vals[i];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) atomic_add_u64(&a.misc[i], v);
+        }
+    }
+}
+
+int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen)
+{
+    size_t lds = (size_t)(2 * a.nlds + kMaxTissues) * sizeof(float4);
+    hipError_t e;
+    if (a.rng_mode == 1) {
+        lds += sizeof(Ran3Lds);
+        e = hipFuncSetAttribute((const void*)transport_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(transport_kernel<1>, dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
+    } else {
+        e = hipFuncSetAttribute((const void*)transport_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(transport_kernel<0>, dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "transport launch: %s", hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// element-wise access to the math core, for the parity tests
+// ---------------------------------------------------------------------------
+__global__ void debug_math_kernel(int op, const float* in, float* out, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = in[2 * i], y = in[2 * i + 1];
+    float r;
+    switch (op) {
+    case 0: r = __builtin_sqrtf(x); break;
+    case 1: r = x / y; break;
+    case 2: r = logf_lp(x); break;
+    case 3: r = cosf_lp(x); break;
+    case 4: r = acosf_lp(x); break;
+    case 5: r = x * y; break;
+    default: r = x + y; break;
+    }
+    out[i] = r;
+}
+
+int launch_debug_math(int op, const float* in, float* out, int64_t n, void* stream)
+{
+    if (n <= 0) return 0;
+    int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(debug_math_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, op, in, out, n);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace lp

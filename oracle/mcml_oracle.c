@@ -505,7 +505,7 @@ static void record_absorption(const orc_params *P, const orc_layout *L, orc_tall
         float fz = (c.z - P->vox_origin[2]) / P->vox_size[2];
         int ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f;
         int32_t ix = orc_cvt(fx), iy = orc_cvt(fy), iz = orc_cvt(fz);
-        ok = ok && ix < P->vox_n[0] && iy < P->vox_n[1] && iz < P->vox_n[2];
+        ok = ok && ix >= 0 && iy >= 0 && iz >= 0 && ix < P->vox_n[0] && iy < P->vox_n[1] && iz < P->vox_n[2];
         if (ok)
             add64(&T->vox[((int64_t)iz * P->vox_n[1] + iy) * P->vox_n[0] + ix], q);
         else
@@ -711,7 +711,8 @@ static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc
         steps++;
         if (trace && ntr < trace_max) {
             float *t = trace + 12 * (int64_t)ntr;
-            t[0] = (float)kind; t[1] = (float)tri; t[2] = (float)curr; t[3] = w;
+            t[0] = (float)kind; t[1] = (float)(kind == ORC_EV_INTERACT ? -1 : tri);
+            t[2] = (float)curr; t[3] = w;
             t[4] = p.x; t[5] = p.y; t[6] = p.z; t[7] = v.x; t[8] = v.y; t[9] = v.z;
             t[10] = stepsize; t[11] = (float)rng.ndraw;
             ntr++;

@@ -1,0 +1,45 @@
+"""CPU-side checks of the C ABI: liblp.so loads, exports every entry point
+declared in include/lightprop.h, and rejects bad calls without a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import lightpropagation_amd as lp
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            names += re.findall(r"^\s*(?:int|void|const char \*|int32_t|int64_t)\s*\*?\s*(lp\w+)\s*\(", txt, re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    d = declared()
+    assert "lp_run" in d and "lp_set_mesh" in d and len(d) >= 20
+
+
+@pytest.mark.parametrize("name", declared())
+def test_library_exports(name):
+    L = ctypes.CDLL(lp._capi.LIB_PATH)
+    assert hasattr(L, name), name
+
+
+def test_create_without_device_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(lp.LPError):
+        lp.Context(0)
+
+
+def test_null_context_is_rejected():
+    L = lp.lib()
+    assert L.lp_run(None, 0, 1) != 0
+    assert L.lp_last_error(None) == b"null context"

@@ -1,0 +1,147 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bit-exact, on the same seeded photons:
+  * the kernel's math core (sqrt, divide vs IEEE float32; log, cos, acos vs
+    the oracle's implementations of the same series),
+  * per-photon step traces (event kind, face, tissue, w, p, v, step length,
+    random draws consumed) in both RNG modes,
+  * per-photon exit records and every tally (SFT, RTd, Arz, Phi_rz, per
+    region, voxels, radial bins, detectors, counters),
+  * determinism: repeated and split GPU runs give identical tallies.
+Scenes: the reference fixtures model1 (MCML slab), model2 (mismatched n,
+isotropic), model6/configFB (thick slab, n = 1.37), model4 (five concentric
+icospheres, 25 600 faces) and the three-layer slab of the bench config 0.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import lightpropagation_amd as lp
+import _oracle as O
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+TALLIES = dict(vox=dict(origin=[-1.0, -1.0, 0.0], size=[0.02, 0.02, 0.01], n=[100, 100, 120]),
+               rad=dict(dr=0.05, n=200), det_rlo=[0.0, 0.2, 0.5, 1.0], det_rhi=[0.2, 0.5, 1.0, 3.0])
+MAXSTEPS = 200000
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = lp.Context(0)
+    yield c
+    c.close()
+
+
+def _scene(name):
+    if name == "c0slab":
+        return scenes.slab(scenes.C0_LAYERS)
+    if name == "model6FB":
+        return scenes.load_fixture("model6", "configFB.xml")
+    return scenes.load_fixture(name)
+
+
+def _configure(ctx, sc, seed, rng):
+    ctx.set_scene(sc, seed=seed, rng=rng, max_steps=MAXSTEPS)
+    ctx.set_voxel_grid(TALLIES["vox"]["origin"], TALLIES["vox"]["size"], TALLIES["vox"]["n"])
+    ctx.set_radial(TALLIES["rad"]["dr"], TALLIES["rad"]["n"])
+    ctx.set_detectors(TALLIES["det_rlo"], TALLIES["det_rhi"])
+    ctx.set_tallies(record_exit=True)
+    ctx.reset()
+
+
+def test_math_core_bit_exact(ctx):
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.random(100000), np.exp(rng.uniform(-90, 40, 50000))]).astype(np.float32)
+    y = np.exp(rng.uniform(-20, 20, len(x))).astype(np.float32)
+    assert np.array_equal(ctx.debug_math(0, x).view(np.uint32), np.sqrt(x).view(np.uint32))
+    assert np.array_equal(ctx.debug_math(1, x, y).view(np.uint32), (x / y).view(np.uint32))
+    xs = x[:20000]
+    assert np.array_equal(ctx.debug_math(2, xs).view(np.uint32),
+                          np.array([O.logf(v) for v in xs], np.float32).view(np.uint32))
+    ps = (np.float32(6.28318548) * rng.random(20000).astype(np.float32)).astype(np.float32)
+    assert np.array_equal(ctx.debug_math(3, ps).view(np.uint32),
+                          np.array([O.cosf(v) for v in ps], np.float32).view(np.uint32))
+    cs = rng.uniform(-1, 1, 20000).astype(np.float32)
+    assert np.array_equal(ctx.debug_math(4, cs).view(np.uint32),
+                          np.array([O.acosf(v) for v in cs], np.float32).view(np.uint32))
+
+
+CASES = [("model1", 512), ("model2", 512), ("model6FB", 256), ("c0slab", 256), ("model4", 48)]
+
+
+@pytest.mark.parametrize("rng", [lp.RNG_PHILOX, lp.RNG_RAN3], ids=["philox", "ran3"])
+@pytest.mark.parametrize("name,n", CASES, ids=[c[0] for c in CASES])
+def test_traces_bit_exact(ctx, name, n, rng):
+    sc = _scene(name)
+    maxrec = 3000
+    _configure(ctx, sc, 4242, rng)
+    g, gn = ctx.trace(100, n, maxrec)
+    orc = O.Run(sc, seed=4242, rng_mode=rng, max_steps=MAXSTEPS, tallies=TALLIES)
+    _, o, on = orc.run(100, n, trace_max=maxrec, nthreads=8)
+    assert np.array_equal(gn, on), np.nonzero(gn != on)[0][:5]
+    bad = []
+    for i in range(n):
+        k = on[i]
+        if not np.array_equal(g[i, :k].view(np.uint32), o[i, :k].view(np.uint32)):
+            j = int(np.nonzero(np.any(g[i, :k].view(np.uint32) != o[i, :k].view(np.uint32), axis=1))[0][0])
+            bad.append((i, j, g[i, j].tolist(), o[i, j].tolist()))
+    assert not bad, bad[:3]
+
+
+@pytest.mark.parametrize("rng", [lp.RNG_PHILOX, lp.RNG_RAN3], ids=["philox", "ran3"])
+@pytest.mark.parametrize("name,n", [("model1", 20000), ("model2", 8000), ("c0slab", 3000), ("model4", 96)],
+                         ids=["model1", "model2", "c0slab", "model4"])
+def test_tallies_bit_exact(ctx, name, n, rng):
+    sc = _scene(name)
+    _configure(ctx, sc, 99, rng)
+    ctx.run(0, n)
+    ex = ctx.exit_records(n)
+    gt = ctx.tallies()
+    orc = O.Run(sc, seed=99, rng_mode=rng, max_steps=MAXSTEPS, tallies=TALLIES)
+    oex, _, _ = orc.run(0, n, exit_rec=True, nthreads=8)
+    ot = orc.tallies()
+    assert np.array_equal(ex.view(np.uint32), oex.view(np.uint32))
+    for k in ("sft", "rtd", "arz", "phirz", "region", "vox", "rad", "det"):
+        a, b = gt[k], ot[k][:len(gt[k])]
+        assert np.array_equal(a, b), k
+    for k in ("photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out"):
+        assert gt["misc"][k] == ot["misc"][k], (k, gt["misc"][k], ot["misc"][k])
+
+
+def test_gpu_runs_are_reproducible_and_resumable(ctx):
+    sc = _scene("c0slab")
+    _configure(ctx, sc, 5, lp.RNG_PHILOX)
+    ctx.set_tallies(record_exit=False)
+    ctx.run(0, 200000)
+    a = ctx.tallies()
+    ctx.reset()
+    ctx.run(0, 70000)
+    ctx.run(70000, 130000)
+    b = ctx.tallies()
+    for k in ("sft", "rtd", "arz", "phirz", "region", "vox", "rad", "det"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["misc"]["steps"] == b["misc"]["steps"]
+
+
+def test_mcml_slab_on_gpu(ctx):
+    # 10^6 photons of the MCML verification slab: Rd, Tt within 3 sigma of
+    # the published values (see tests/test_oracle_physics.py for sources)
+    sc = _scene("model1")
+    _configure(ctx, sc, 12345, lp.RNG_PHILOX)
+    ctx.set_tallies(record_exit=True)
+    n = 1000000
+    ctx.run(0, n)
+    ex = ctx.exit_records(n).astype(np.float64)
+    R, T = ex[:, 3], ex[:, 5]
+    sR = math.hypot(R.std() / math.sqrt(n), 0.00035)
+    sT = math.hypot(T.std() / math.sqrt(n), 0.00020)
+    assert abs(R.mean() - 0.09734) < 3 * sR
+    assert abs(T.mean() - 0.66096) < 3 * sT
+    t = ctx.tallies()
+    A = t["region"].sum() / lp.FIXED / n
+    assert abs(R.mean() + T.mean() + A - 1.0) < 1e-5
+    # tallies agree with the per-photon records
+    assert abs(t["sft"].reshape(-1, 3)[:, 0].sum() / lp.FIXED - R.sum()) < 1e-3

@@ -50,7 +50,7 @@ extern "C" {
 #define LP_TALLY_MISC 8   /* counters: photons, steps, interactions, boundary
                              events, escaped, capped, off-grid Arz weight,
                              off-grid voxel weight, BVH node visits, triangle
-                             tests, skipped searches (16 slots)               */
+                             tests, skipped searches, NaN photons (16 slots)  */
 #define LP_NTALLY 9
 
 /* Tally values are int64 fixed point: 2^32 per unit photon weight, except

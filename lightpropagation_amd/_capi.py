@@ -15,7 +15,7 @@ LP_OK = 0
 RNG_PHILOX, RNG_RAN3 = 0, 1
 TALLY = {"sft": 0, "rtd": 1, "arz": 2, "phirz": 3, "region": 4, "vox": 5, "rad": 6, "det": 7, "misc": 8}
 MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out",
-        "node_visits", "tri_tests", "skipped"]
+        "node_visits", "tri_tests", "skipped", "nan"]
 FIXED = 2.0 ** 32
 FIXED_PHI = 2.0 ** 24
 
@@ -63,6 +63,17 @@ _SIGS = {
     "lp_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "lp_debug_math": (C.c_int, [C.c_void_p, C.c_int32, _fp, _fp, C.c_int64]),
     "lp_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "lpm_mf_default": (None, [C.c_void_p]),
+    "lpm_mf_num_triangles": (C.c_int64, [C.c_void_p]),
+    "lpm_mf_build": (C.c_int, [C.c_void_p, _fp, _fp, _ip, C.c_int64]),
+    "lpm_mf_optics": (C.c_int, [C.c_double, C.c_double, C.c_double, _fp, C.c_int32]),
+    "lpm_slab_num_triangles": (C.c_int64, [C.c_int32]),
+    "lpm_slab_build": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.c_double, C.c_double, _fp, _fp, _ip,
+                                 C.c_int64]),
+    "lpm_merge_coincident": (C.c_int64, [_fp, _fp, _ip, C.c_int64]),
+    "lpm_box": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32, _fp, _fp, _ip]),
+    "lpm_emitter": (C.c_int, [_fp, _fp, C.c_float, _fp, _fp]),
+    "lpm_cell_normals": (C.c_int, [_fp, C.c_int64, _fp]),
 }
 
 

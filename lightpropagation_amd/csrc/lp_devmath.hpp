@@ -38,6 +38,11 @@ __device__ __forceinline__ f3 normalize(f3 p)
     float inv = 1.0f / __builtin_sqrtf(d);
     return mul(inv, p);
 }
+// 1 - c^2 of two unit vectors is negative only by rounding; the reference
+// takes the square root anyway (MCLightPropagation.cl:520, 541, 880) and the
+// photon turns NaN.  Clamped at 0 here; finite trajectories are unchanged.
+__device__ __forceinline__ float clamp0(float q) { return q < 0.0f ? 0.0f : q; }
+
 __device__ __forceinline__ float sign_cl(float x)
 {
     if (x > 0.0f) return 1.0f;

@@ -108,7 +108,7 @@ struct KArgs {
 
 enum MiscSlot {
     kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
-    kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscN = 16
+    kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscN = 16
 };
 
 // Launchers (defined in lp_kernel.hip).

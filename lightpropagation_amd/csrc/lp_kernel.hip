@@ -160,6 +160,11 @@ __device__ __forceinline__ void find_face(const KArgs& a, const float4* lnodes, 
     h.id = -1;
     h.med = 0;
     if (a.nnodes == 0) return;
+    // A non-finite ray meets no face (every exact test compares NaN); return
+    // at once rather than let NaN slab tests open every box.
+    if (!(__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z) &&
+          __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z)))
+        return;
     const float ix = (v.x != 0.0f) ? 1.0f / v.x : 3.0e38f;
     const float iy = (v.y != 0.0f) ? 1.0f / v.y : 3.0e38f;
     const float iz = (v.z != 0.0f) ? 1.0f / v.z : 3.0e38f;
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
     bool alive = false;
     bool exhausted = false;  // wave-uniform
     uint64_t c_steps = 0, c_int = 0, c_bnd = 0, c_esc = 0, c_cap = 0, c_phot = 0, c_skip = 0;
-    uint64_t c_arz_out = 0, c_vox_out = 0;
+    uint64_t c_arz_out = 0, c_vox_out = 0, c_nan = 0;
     uint32_t c_visits = 0, c_tests = 0;
     uint64_t c_visits64 = 0, c_tests64 = 0;
     unsigned long long reg_acc = 0;  // fixed-point absorption in ph.curr not yet flushed
@@ -397,13 +402,13 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
                     r = 1.0f;
                 } else if (n1 == n2) {  // GetFresnel (.cl:857-907)
                     r = 0.0f;
-                } else if (abs_cos == 1.0f) {
+                } else if (abs_cos >= 1.0f) {  // normal incidence (> 1 only by rounding)
                     r = (n1 - n2) / (n1 + n2);
                     r *= r;
                 } else if (cos_in == 0.0f) {
                     r = 1.0f;
                 } else {
-                    float sin_in = __builtin_sqrtf(1 - cos_in * cos_in);
+                    float sin_in = __builtin_sqrtf(clamp0(1 - cos_in * cos_in));
                     float sin_out = n1 * sin_in / n2;
                     if (sin_out >= 1.0f) {
                         r = 1.0f;
@@ -420,8 +425,8 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
                 int32_t new_id;
                 float m_a, m_s;
                 if (x > r) {  // Transmit (.cl:506-529)
-                    float sin_t = __builtin_sqrtf(1 - cos_in * cos_in) * n1 / n2;
-                    float cos_t = -__builtin_sqrtf(1 - sin_t * sin_t);
+                    float sin_t = __builtin_sqrtf(clamp0(1 - cos_in * cos_in)) * n1 / n2;
+                    float cos_t = -__builtin_sqrtf(clamp0(1 - sin_t * sin_t));
                     if (cos_in > 0.0f) cos_t = __builtin_fabsf(cos_t);
                     ph.v = normalize(sub(mul(cos_t, ph.nv), mul(sin_t, ey)));
                     new_id = (cos_in > 0) ? out_id : in_id;
@@ -429,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
                     m_s = nm.z;
                     kind = 1;
                 } else {  // Reflect (.cl:532-549)
-                    float sin_r = __builtin_sqrtf(1 - cos_in * cos_in);
+                    float sin_r = __builtin_sqrtf(clamp0(1 - cos_in * cos_in));
                     ph.v = neg(normalize(add(mul(cos_in, ph.nv), mul(sin_r, ey))));
                     new_id = ph.curr;
                     m_a = cm.y;
@@ -574,6 +579,9 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
 
         // ---- epilogue (.cl:1586-1594) + host binning (cpp:1553-1630) ----
         alive = false;
+        if (ph.p.x != ph.p.x || ph.p.y != ph.p.y || ph.p.z != ph.p.z || ph.v.x != ph.v.x || ph.v.y != ph.v.y ||
+            ph.v.z != ph.v.z)
+            c_nan++;
         if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
         reg_acc = 0;
         float r8[8];
@@ -646,8 +654,8 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
     c_tests64 += c_tests;
     if (a.misc) {
         uint64_t vals[kMiscN] = {c_phot, c_steps, c_int, c_bnd, c_esc, c_cap, c_arz_out, c_vox_out,
-                                 c_visits64, c_tests64, c_skip, 0, 0, 0, 0, 0};
-        for (int i = 0; i < 11; ++i) {
+                                 c_visits64, c_tests64, c_skip, c_nan, 0, 0, 0, 0};
+        for (int i = 0; i < 12; ++i) {
             uint64_t v = vals[i];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
             if (lane == 0) atomic_add_u64(&a.misc[i], v);

@@ -60,6 +60,13 @@ static inline of3 vnormalize(of3 p)
 }
 static inline of3 vload3(const float *a, int64_t i) { return V(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
 
+/* 1 - c^2 is negative only when |c| exceeds 1 by rounding (two unit
+ * vectors whose squared norms round above 1).  The reference takes the
+ * square root anyway (MCLightPropagation.cl:520, 541, 880) and the photon
+ * continues as NaN; here the argument is clamped at 0, which leaves every
+ * finite trajectory unchanged. */
+static inline float orc_clamp0(float q) { return q < 0.0f ? 0.0f : q; }
+
 float orc_sign(float x)
 {
     if (x > 0.0f) return 1.0f;
@@ -354,13 +361,13 @@ float orc_fresnel(float n1, float n2, float cos_in)
     float r;
     if (n1 == n2) {
         r = 0.0f;
-    } else if (fabsf(cos_in) == 1.0f) {
+    } else if (fabsf(cos_in) >= 1.0f) { /* normal incidence (|cos| > 1 only by rounding) */
         r = (n1 - n2) / (n1 + n2);
         r *= r;
     } else if (cos_in == 0.0f) {
         r = 1.0f;
     } else {
-        float sin_in = sqrtf(1 - cos_in * cos_in);
+        float sin_in = sqrtf(orc_clamp0(1 - cos_in * cos_in));
         float sin_out = n1 * sin_in / n2;
         if (sin_out >= 1.0f) {
             r = 1.0f;
@@ -651,15 +658,15 @@ static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc
                 float m_a, m_s;
                 of3 ey = vnormalize(vcross(nv, vcross(nv, v)));
                 if (x > r) { /* Transmit (.cl:506-529) */
-                    float sin_t = sqrtf(1 - cos_in * cos_in) * n1 / n2;
-                    float cos_t = -sqrtf(1 - sin_t * sin_t);
+                    float sin_t = sqrtf(orc_clamp0(1 - cos_in * cos_in)) * n1 / n2;
+                    float cos_t = -sqrtf(orc_clamp0(1 - sin_t * sin_t));
                     if (cos_in > 0.0f) cos_t = fabsf(cos_t);
                     v = vnormalize(vsub(vmul(cos_t, nv), vmul(sin_t, ey)));
                     new_id = (cos_in > 0) ? out_id : in_id;
                     m_a = n_a; m_s = n_s;
                     kind = ORC_EV_TRANSMIT;
                 } else {    /* Reflect (.cl:532-549) */
-                    float sin_r = sqrtf(1 - cos_in * cos_in);
+                    float sin_r = sqrtf(orc_clamp0(1 - cos_in * cos_in));
                     v = vneg(vnormalize(vadd(vmul(cos_in, nv), vmul(sin_r, ey))));
                     new_id = curr;
                     m_a = c_a; m_s = c_s;
@@ -722,6 +729,8 @@ static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc
             w = 0.0f;
         }
     } while (w > 0.0f && (w > P->w_th || roulette));
+    if (p.x != p.x || p.y != p.y || p.z != p.z || v.x != v.x || v.y != v.y || v.z != v.z)
+        misc_loc[ORC_MISC_NAN]++;
     misc_loc[ORC_MISC_STEPS] += steps;
     misc_loc[ORC_MISC_ARZ_OUT] += arz_out;
     misc_loc[ORC_MISC_VOX_OUT] += vox_out;

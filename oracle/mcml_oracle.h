@@ -58,7 +58,8 @@ extern "C" {
 #define ORC_MISC_CAPPED 5       /* photons stopped by the step cap          */
 #define ORC_MISC_ARZ_OUT 6      /* fixed-point weight deposited off the Arz grid */
 #define ORC_MISC_VOX_OUT 7      /* fixed-point weight deposited off the voxel grid */
-#define ORC_MISC_N 8
+#define ORC_MISC_NAN 8          /* photons whose position or direction ended NaN */
+#define ORC_MISC_N 9
 
 typedef struct orc_params {
     /* mesh: triangle t has vertices tri[9t..9t+8], normal nrm[3t..], and

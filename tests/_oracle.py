@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 RNG_PHILOX = 0
 RNG_RAN3F = 1
 EV_INTERACT, EV_TRANSMIT, EV_REFLECT, EV_ESCAPE = 0, 1, 2, 3
-MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out"]
+MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out", "nan"]
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
@@ -46,7 +46,7 @@ class Layout(C.Structure):
 
 class Tallies(C.Structure):
     _fields_ = [(n, _lp) for n in ("sft", "rtd", "arz", "phirz", "region", "vox", "rad", "det")] + \
-               [("misc", C.c_int64 * 8)]
+               [("misc", C.c_int64 * 9)]
 
 
 _lib = None

@@ -143,3 +143,13 @@ def test_hg_ks():
     cdf = (1 - g * g) / (2 * g) * (1 / np.sqrt(1 + g * g - 2 * g * mu) - 1 / (1 + g))
     d = np.max(np.abs(cdf - (np.arange(1, len(cs) + 1) / len(cs))))
     assert d < 1.63 / math.sqrt(len(cs))  # 1% level
+
+
+def test_fresnel_rounded_normal_incidence_is_finite():
+    # |cos| slightly above 1 (two unit vectors whose squared norms round above
+    # 1): the normal-incidence value, not NaN (the reference takes sqrt of a
+    # negative number here and the photon continues as NaN)
+    for n1, n2 in [(1.4, 1.0), (1.0, 1.4), (1.4, 1.4)]:
+        for c in (1.0000001, -1.0000001):
+            r = O.fresnel(n1, n2, c)
+            assert abs(r - ((n1 - n2) / (n1 + n2)) ** 2) < 1e-7

@@ -15,7 +15,7 @@ LP_OK = 0
 RNG_PHILOX, RNG_RAN3 = 0, 1
 TALLY = {"sft": 0, "rtd": 1, "arz": 2, "phirz": 3, "region": 4, "vox": 5, "rad": 6, "det": 7, "misc": 8}
 MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out",
-        "node_visits", "tri_tests", "skipped", "nan"]
+        "node_visits", "tri_tests", "skipped", "nan", "dist_queries", "dist_visits"]
 FIXED = 2.0 ** 32
 FIXED_PHI = 2.0 ** 24
 

@@ -56,6 +56,8 @@ struct lp_context {
     uint32_t tally_mask = 0xffffffffu;
     int32_t record_exit = 0;
     int32_t grid_override = 0, lds_override = -1, safety_cells = -1;
+    float dist_cap = 1.0f, dist_cap_steps = 64.0f;
+    float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
 
     // derived
     lp_layout L{};
@@ -209,6 +211,8 @@ int prepare(lp_context* c)
     HIPCHK(c, hipSetDevice(c->device));
     if (c->mesh_dirty) {
         c->bvh = build_bvh(c->tri.data(), c->med.data(), c->ntri, 4);
+        c->bvh_extent = 0.0f;
+        for (float x : c->tri) c->bvh_extent = std::max(c->bvh_extent, std::fabs(x));
         int cells = c->safety_cells < 0 ? 128 : c->safety_cells;
         c->sg = cells > 0 ? build_safety_grid(c->bvh, c->tri.data(), c->ntri, cells) : SafetyGrid();
         if ((rc = upload(c, c->d_tris, c->bvh.tris.data(), c->bvh.tris.size() * sizeof(TriRec)))) return rc;
@@ -249,6 +253,9 @@ void fill_args(lp_context* c, KArgs& a)
     a.nrm = (const float*)c->d_nrm.p;
     a.med = (const int32_t*)c->d_med.p;
     a.ntri = (int32_t)c->ntri;
+    a.dist_cap = c->dist_cap;
+    a.dist_cap_steps = c->dist_cap_steps;
+    a.safe_margin = 2e-6f * c->bvh_extent;
     if (!c->sg.dist.empty()) {
         a.safety = (const float*)c->d_safety.p;
         for (int k = 0; k < 3; ++k) {

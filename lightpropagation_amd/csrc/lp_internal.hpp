@@ -73,6 +73,9 @@ struct KArgs {
     const float* safety;  // optional safety grid
     float sg_origin[3], sg_inv[3];
     int32_t sg_n[3];
+    float dist_cap;        // per-photon free-radius search: cap (0 = off)
+    float dist_cap_steps;  // ... and cap in units of the current step length
+    float safe_margin;     // absolute rounding margin of the free radius
     float nmuasg[4 * kMaxTissues];
     int32_t ntissue, fetal_id;
     float pstart[3];
@@ -108,7 +111,8 @@ struct KArgs {
 
 enum MiscSlot {
     kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
-    kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscN = 16
+    kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscDistQueries,
+    kMiscDistVisits, kMiscN = 16
 };
 
 // Launchers (defined in lp_kernel.hip).

@@ -36,7 +36,7 @@ struct Ran3Lds {
 // random numbers
 // ---------------------------------------------------------------------------
 struct PhiloxState {
-    uint32_t k0, k1, id0, id1, block;
+    uint32_t k0, k1, id0, id1, block;  // k0, k1: the (uniform) seed
     uint32_t buf[4];
     int used;
     __device__ void init(uint64_t seed, uint64_t photon)
@@ -118,16 +118,16 @@ __device__ __forceinline__ float ran3f_next(int32_t* m, int tid, int& inext, int
     return r / 1000000000.0f;
 }
 
-template <int RNG>
+template <int RNG, bool COUNT>
 struct Rng {
     PhiloxState ph;
     int inext, inextp;
     int32_t* s;
     int tid;
-    int64_t ndraw;
+    int64_t ndraw;  // draws consumed (kept for the step trace only)
     __device__ void init(uint64_t seed, uint64_t photon)
     {
-        ndraw = 0;
+        if (COUNT) ndraw = 0;
         if (RNG == 0)
             ph.init(seed, photon);
         else
@@ -135,7 +135,7 @@ struct Rng {
     }
     __device__ __forceinline__ float next()
     {
-        ndraw++;
+        if (COUNT) ndraw++;
         if (RNG == 0) return ph.next();
         return ran3f_next(s, tid, inext, inextp);
     }
@@ -235,6 +235,82 @@ __device__ __forceinline__ float safety_at(const KArgs& a, f3 p)
     return a.safety[((int64_t)iz * a.sg_n[1] + iy) * a.sg_n[0] + ix];
 }
 
+// Squared distance from p to the triangle (a, a + ab, a + ac): the
+// closest-point construction of C. Ericson, Real-Time Collision Detection
+// (2005), section 5.1.5, by Voronoi regions of the vertices and edges.
+__device__ __forceinline__ float point_tri_dist2(f3 p, f3 a, f3 ab, f3 ac)
+{
+    f3 ap = sub(p, a);
+    float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) return dot(ap, ap);
+    f3 bp = sub(ap, ab);
+    float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) return dot(bp, bp);
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        f3 q = sub(ap, mul(d1 / (d1 - d3), ab));
+        return dot(q, q);
+    }
+    f3 cp = sub(ap, ac);
+    float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) return dot(cp, cp);
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        f3 q = sub(ap, mul(d2 / (d2 - d6), ac));
+        return dot(q, q);
+    }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        f3 q = sub(bp, mul(w, sub(ac, ab)));
+        return dot(q, q);
+    }
+    float den = 1.0f / (va + vb + vc);
+    f3 q = sub(ap, add(mul(vb * den, ab), mul(vc * den, ac)));
+    return dot(q, q);
+}
+
+// Distance from p to the nearest face, searched out to `cap` (returns cap
+// when nothing is nearer).  Boxes are padded, so box distances are lower
+// bounds; the caller subtracts a rounding margin.
+__device__ __forceinline__ float mesh_distance(const KArgs& a, const float4* lnodes, f3 p, float cap,
+                                               uint32_t& visits)
+{
+    float best2 = cap * cap;
+    const float4* gn = reinterpret_cast<const float4*>(a.nodes);
+    int32_t n = 0;
+    while (n >= 0) {
+        float4 lo, hi;
+        if (n < a.nlds) {
+            lo = lnodes[2 * n];
+            hi = lnodes[2 * n + 1];
+        } else {
+            lo = gn[2 * n];
+            hi = gn[2 * n + 1];
+        }
+        visits++;
+        float dx = fmaxf(fmaxf(lo.x - p.x, p.x - hi.x), 0.0f);
+        float dy = fmaxf(fmaxf(lo.y - p.y, p.y - hi.y), 0.0f);
+        float dz = fmaxf(fmaxf(lo.z - p.z, p.z - hi.z), 0.0f);
+        int32_t hit = __float_as_int(lo.w), miss = __float_as_int(hi.w);
+        if ((dx * dx + dy * dy) + dz * dz < best2) {
+            if (hit >= 0) {
+                n = hit;
+                continue;
+            }
+            int32_t first = hit & 0xffffff, cnt = (hit >> 24) & 0x7f;
+            const float4* tr = reinterpret_cast<const float4*>(a.tris) + 3 * (int64_t)first;
+            for (int32_t j = 0; j < cnt; ++j) {
+                float4 A = tr[3 * j], B = tr[3 * j + 1], C = tr[3 * j + 2];
+                float d2 = point_tri_dist2(p, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z));
+                best2 = fminf(best2, d2);
+            }
+        }
+        n = miss;
+    }
+    return __builtin_sqrtf(best2);
+}
+
 __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v)
 {
     if (v) atomicAdd(p, v);
@@ -246,13 +322,14 @@ __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned l
 struct Photon {
     f3 p, v, nv, sft;
     float w, unfin, absorbed;
+    float safe;  // no face lies within this distance of p
     int32_t ref, curr;
     int32_t fetal, roulette;
     int64_t steps;
     int64_t k;  // index within this run
 };
 
-template <int RNG>
+template <int RNG, bool TRACE>
 __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
 {
     extern __shared__ float4 smem[];
@@ -276,31 +353,38 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
     const f3 eZ = mk(a.ez[0], a.ez[1], a.ez[2]);
     const f3 eZr = mk(a.ez_raw[0], a.ez_raw[1], a.ez_raw[2]);
 
-    Rng<RNG> rng;
+    Rng<RNG, TRACE> rng;
     rng.s = lran3;
     rng.tid = tid;
     Photon ph;
     bool alive = false;
     bool exhausted = false;  // wave-uniform
-    uint64_t c_steps = 0, c_int = 0, c_bnd = 0, c_esc = 0, c_cap = 0, c_phot = 0, c_skip = 0;
-    uint64_t c_arz_out = 0, c_vox_out = 0, c_nan = 0;
-    uint32_t c_visits = 0, c_tests = 0;
-    uint64_t c_visits64 = 0, c_tests64 = 0;
+    // per-lane 32-bit counters (photons, steps, interactions, boundary
+    // events, escaped, capped, skipped searches, distance queries, NaN
+    // photons, BVH visits, triangle tests, distance-query visits); flushed
+    // to the 64-bit totals when one grows large and at the end
+    enum { kPhot, kSteps, kInt, kBnd, kEsc, kCap, kSkip, kDq, kNan, kVis, kTst, kDvis, kNc };
+    uint32_t cnt[kNc];
+#pragma unroll
+    for (int i = 0; i < kNc; ++i) cnt[i] = 0;
+    uint64_t c_arz_out = 0, c_vox_out = 0;
     unsigned long long reg_acc = 0;  // fixed-point absorption in ph.curr not yet flushed
+    int64_t vox_cell = -1;           // voxel whose deposits are pending in vox_acc
+    unsigned long long vox_acc = 0;
 
     for (;;) {
         // ---- refill lanes whose photon has finished (one atomic per wave) ----
         if (!exhausted) {
             unsigned long long need = __ballot(!alive);
             if (need) {
-                int cnt = __popcll(need);
+                int nneed = __popcll(need);
                 int leader = __ffsll((long long)need) - 1;
                 unsigned long long got = 0;
-                if (lane == leader) got = atomicAdd(a.counter, (unsigned long long)cnt);
+                if (lane == leader) got = atomicAdd(a.counter, (unsigned long long)nneed);
                 uint32_t blo = __shfl((uint32_t)got, leader), bhi = __shfl((uint32_t)(got >> 32), leader);
                 unsigned long long base = ((unsigned long long)bhi << 32) | blo;
                 int rank = __popcll(need & ((1ull << lane) - 1ull));
-                if (base + (unsigned long long)cnt >= (unsigned long long)a.count) exhausted = true;
+                if (base + (unsigned long long)nneed >= (unsigned long long)a.count) exhausted = true;
                 if (!alive) {
                     unsigned long long k = base + (unsigned long long)rank;
                     if (k < (unsigned long long)a.count) {
@@ -329,10 +413,11 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
                         ph.nv = mk(0.0f, 0.0f, -1.0f);
                         ph.roulette = 1;
                         ph.steps = 0;
+                        ph.safe = 0.0f;
                         ph.k = (int64_t)k;
                         reg_acc = 0;
                         alive = true;
-                        c_phot++;
+                        cnt[kPhot]++;
                     }
                 }
             }
@@ -357,11 +442,28 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
             }
         }
         Hit h;
-        if (!ballistic && stepsize < safety_at(a, ph.p)) {
+        bool skip = false;
+        if (!ballistic) {
+            if (stepsize < ph.safe || stepsize < safety_at(a, ph.p)) {
+                skip = true;
+            } else if (a.dist_cap > 0.0f && ph.ref < 0 && __builtin_isfinite(ph.p.x) &&
+                       __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z)) {
+                // refresh the photon's free radius: nearest face distance less
+                // a rounding margin (a face found by the exact test at t <= s
+                // lies within s + margin of p)
+                float cap = fminf(a.dist_cap, a.dist_cap_steps * stepsize);
+                float d = mesh_distance(a, lnodes, ph.p, cap, cnt[kDvis]);
+                float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)), __builtin_fabsf(ph.p.z));
+                ph.safe = d - (a.safe_margin + 1e-5f * (d + rad));
+                cnt[kDq]++;
+                skip = stepsize < ph.safe;
+            }
+        }
+        if (skip) {
             h.id = -1;
-            c_skip++;
+            cnt[kSkip]++;
         } else {
-            find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, c_visits, c_tests);
+            find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, cnt[kVis], cnt[kTst]);
         }
         float stb = 10000.0f;
         bool boundary;
@@ -376,10 +478,11 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
         if (boundary) {
             ph.p = add(ph.p, mul(stb, ph.v));  // MoveStep
             ph.ref = -1;
+            ph.safe = 0.0f;
             if (h.id < 0) {  // nothing ahead: the photon has left the mesh
                 kind = 3;
                 ph.w = 0.0f;
-                c_esc++;
+                cnt[kEsc]++;
             } else {
                 float x = rng.next();
                 int32_t in_id = h.med & 0xffff, out_id = (h.med >> 16) & 0xffff;
@@ -462,11 +565,12 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
                 if (ph.curr == a.fetal_id) ph.fetal = 1;
                 ph.unfin = (stepsize - stb) * (cm.y + cm.z);
             }
-            c_bnd++;
+            cnt[kBnd]++;
         } else {
             // interaction branch (.cl:1513-1537)
             ph.p = add(ph.p, mul(stepsize, ph.v));
             ph.ref = -1;
+            ph.safe = ph.safe - stepsize;
             if (cm.y > 0.0f && cm.z > 0.0f) {  // Absorb (.cl:841-847)
                 ph.absorbed = ph.w * (cm.y / (cm.y + cm.z));
                 ph.w -= ph.absorbed;
@@ -535,12 +639,20 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
                 bool ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f;
                 int32_t ix = cvt_i32f(fx), iy = cvt_i32f(fy), iz = cvt_i32f(fz);
                 ok = ok && ix >= 0 && iy >= 0 && iz >= 0 && ix < a.vox_n[0] && iy < a.vox_n[1] && iz < a.vox_n[2];
-                if (ok)
-                    atomic_add_u64(&a.vox[((int64_t)iz * a.vox_n[1] + iy) * a.vox_n[0] + ix], q);
-                else
+                if (ok) {
+                    // consecutive deposits in one voxel are summed in a register
+                    int64_t cell = ((int64_t)iz * a.vox_n[1] + iy) * a.vox_n[0] + ix;
+                    if (cell != vox_cell) {
+                        if (vox_cell >= 0) atomic_add_u64(&a.vox[vox_cell], vox_acc);
+                        vox_cell = cell;
+                        vox_acc = 0;
+                    }
+                    vox_acc += q;
+                } else {
                     c_vox_out += q;
+                }
             }
-            c_int++;
+            cnt[kInt]++;
             kind = 0;
         }
         // Roulette (.cl:1020-1035, 1539-1545)
@@ -555,8 +667,8 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
             }
         }
         ph.steps++;
-        c_steps++;
-        if (a.trace && ph.steps <= a.trace_max) {
+        cnt[kSteps]++;
+        if (TRACE && ph.steps <= a.trace_max) {
             float* t = a.trace + ((int64_t)ph.k * a.trace_max + (ph.steps - 1)) * 12;
             t[0] = (float)kind;
             t[1] = (float)(boundary ? h.id : -1);
@@ -569,10 +681,10 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
             t[8] = ph.v.y;
             t[9] = ph.v.z;
             t[10] = stepsize;
-            t[11] = (float)rng.ndraw;
+            t[11] = (float)(TRACE ? rng.ndraw : 0);
         }
         if (a.max_steps > 0 && ph.steps >= a.max_steps && ph.w > 0.0f) {
-            c_cap++;
+            cnt[kCap]++;
             ph.w = 0.0f;
         }
         if (ph.w > 0.0f && (ph.w > a.w_th || ph.roulette)) continue;
@@ -581,34 +693,30 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
         alive = false;
         if (ph.p.x != ph.p.x || ph.p.y != ph.p.y || ph.p.z != ph.p.z || ph.v.x != ph.v.x || ph.v.y != ph.v.y ||
             ph.v.z != ph.v.z)
-            c_nan++;
+            cnt[kNan]++;
         if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
         reg_acc = 0;
-        float r8[8];
-        {
-            const f3 d = sub(ph.p, POI);
-            const f3 dn = sub(POI, ph.p);
-            r8[0] = dot(eX, d);
-            r8[1] = dot(eY, d);
-            r8[2] = dot(eZ, d);
-            r8[3] = ph.sft.x;
-            r8[4] = ph.sft.y;
-            r8[5] = ph.sft.z;
-            r8[6] = __builtin_fabsf(acosf_lp(dot(ph.v, ph.nv)));
-            r8[7] = __builtin_sqrtf((dn.x * dn.x + dn.y * dn.y) + dn.z * dn.z);
-        }
+        if (vox_cell >= 0) atomic_add_u64(&a.vox[vox_cell], vox_acc);
+        vox_cell = -1;
+        vox_acc = 0;
+        const f3 dq = sub(ph.p, POI);
+        const f3 dn = sub(POI, ph.p);
+        const float e0 = dot(eX, dq), e1 = dot(eY, dq), e2 = dot(eZ, dq);
+        const float e3 = ph.sft.x, e4 = ph.sft.y, e5 = ph.sft.z;
+        const float e6 = __builtin_fabsf(acosf_lp(dot(ph.v, ph.nv)));
+        const float e7 = __builtin_sqrtf((dn.x * dn.x + dn.y * dn.y) + dn.z * dn.z);
         if (a.exit_rec) {
             float4* o = reinterpret_cast<float4*>(a.exit_rec + 8 * ph.k);
-            o[0] = make_float4(r8[0], r8[1], r8[2], r8[3]);
-            o[1] = make_float4(r8[4], r8[5], r8[6], r8[7]);
+            o[0] = make_float4(e0, e1, e2, e3);
+            o[1] = make_float4(e4, e5, e6, e7);
         }
-        if (a.trace_n) a.trace_n[ph.k] = (int32_t)(ph.steps < a.trace_max ? ph.steps : a.trace_max);
-        if (r8[3] != 0.0f || r8[4] != 0.0f || r8[5] != 0.0f) {
-            unsigned long long q0 = fx32(r8[3]), q1 = fx32(r8[4]), q2 = fx32(r8[5]);
+        if (TRACE) a.trace_n[ph.k] = (int32_t)(ph.steps < a.trace_max ? ph.steps : a.trace_max);
+        if (e3 != 0.0f || e4 != 0.0f || e5 != 0.0f) {
+            unsigned long long q0 = fx32(e3), q1 = fx32(e4), q2 = fx32(e5);
             if (a.sft) {
                 int32_t ex0 = -(a.dimx / 2), ex1 = a.dimx / 2, ey0 = -(a.dimy / 2), ey1 = a.dimy / 2;
-                int32_t x = cvt_i32f(r8[0] / a.det_grid[0]);
-                int32_t y = cvt_i32f(r8[1] / a.det_grid[1]);
+                int32_t x = cvt_i32f(e0 / a.det_grid[0]);
+                int32_t y = cvt_i32f(e1 / a.det_grid[1]);
                 y = (y < ey0) ? ey0 : y;
                 y = (y > ey1) ? ey1 : y;
                 if (x < ex0) x = ex0;
@@ -619,8 +727,8 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
                 atomic_add_u64(&a.sft[px + 2], q2);
             }
             if (a.rtd) {
-                int32_t ia = cvt_i32(((double)(r8[6] * 180.0f) / 3.14159265358979323846) / (double)a.det_grid[5]);
-                int32_t ir = cvt_i32f(r8[7] / a.det_grid[3]);
+                int32_t ia = cvt_i32(((double)(e6 * 180.0f) / 3.14159265358979323846) / (double)a.det_grid[5]);
+                int32_t ir = cvt_i32f(e7 / a.det_grid[3]);
                 if (ir > a.dimR - 1) ir = a.dimR - 1;
                 else if (ir < 0) ir = 0;
                 if (ia > a.dimAlpha - 1) ia = a.dimAlpha - 1;
@@ -631,7 +739,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
                 atomic_add_u64(&a.rtd[px + 2], q2);
             }
             if (a.rad) {
-                int32_t ib = cvt_i32f(r8[7] / a.rad_dr);
+                int32_t ib = cvt_i32f(e7 / a.rad_dr);
                 if (ib >= 0 && ib < a.rad_n) {
                     atomic_add_u64(&a.rad[3 * ib], q0);
                     atomic_add_u64(&a.rad[3 * ib + 1], q1);
@@ -640,25 +748,44 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
             }
             if (a.det) {
                 for (int kd = 0; kd < a.ndet; ++kd) {
-                    if (r8[7] >= a.det_rlo[kd] && r8[7] < a.det_rhi[kd]) {
+                    if (e7 >= a.det_rlo[kd] && e7 < a.det_rhi[kd]) {
                         atomic_add_u64(&a.det[2 * kd], q0);
                         atomic_add_u64(&a.det[2 * kd + 1], q1);
                     }
                 }
             }
         }
-        if (c_visits > 0x40000000u) { c_visits64 += c_visits; c_visits = 0; }
-        if (c_tests > 0x40000000u) { c_tests64 += c_tests; c_tests = 0; }
+        bool big = false;
+#pragma unroll
+        for (int i = 0; i < kNc; ++i) big |= cnt[i] > 0x40000000u;
+        if (big && a.misc) {  // rare: keep the 32-bit counters from wrapping
+            const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
+                                   kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
+                                   kMiscDistVisits};
+#pragma unroll
+            for (int i = 0; i < kNc; ++i) {
+                atomic_add_u64(&a.misc[slot[i]], cnt[i]);
+                cnt[i] = 0;
+            }
+        }
     }
-    c_visits64 += c_visits;
-    c_tests64 += c_tests;
     if (a.misc) {
-        uint64_t vals[kMiscN] = {c_phot, c_steps, c_int, c_bnd, c_esc, c_cap, c_arz_out, c_vox_out,
-                                 c_visits64, c_tests64, c_skip, c_nan, 0, 0, 0, 0};
-        for (int i = 0; i < 12; ++i) {
-            uint64_t v = vals[i];
+        const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
+                               kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
+                               kMiscDistVisits};
+#pragma unroll
+        for (int i = 0; i < kNc; ++i) {
+            uint64_t v = cnt[i];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) atomic_add_u64(&a.misc[i], v);
+            if (lane == 0) atomic_add_u64(&a.misc[slot[i]], v);
+        }
+        uint64_t w2[2] = {c_arz_out, c_vox_out};
+        const int slot2[2] = {kMiscArzOut, kMiscVoxOut};
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            uint64_t v = w2[i];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) atomic_add_u64(&a.misc[slot2[i]], v);
         }
     }
 }
@@ -667,17 +794,22 @@ int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errl
 {
     size_t lds = (size_t)(2 * a.nlds + kMaxTissues) * sizeof(float4);
     hipError_t e;
-    if (a.rng_mode == 1) {
-        lds += sizeof(Ran3Lds);
-        e = hipFuncSetAttribute((const void*)transport_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-        if (e == hipSuccess)
-            hipLaunchKernelGGL(transport_kernel<1>, dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
-    } else {
-        e = hipFuncSetAttribute((const void*)transport_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-        if (e == hipSuccess)
-            hipLaunchKernelGGL(transport_kernel<0>, dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
+    const void* fn;
+    if (a.rng_mode == 1) lds += sizeof(Ran3Lds);
+    if (a.rng_mode == 1)
+        fn = a.trace ? (const void*)transport_kernel<1, true> : (const void*)transport_kernel<1, false>;
+    else
+        fn = a.trace ? (const void*)transport_kernel<0, true> : (const void*)transport_kernel<0, false>;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) {
+        if (a.rng_mode == 1 && a.trace)
+            hipLaunchKernelGGL((transport_kernel<1, true>), dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
+        else if (a.rng_mode == 1)
+            hipLaunchKernelGGL((transport_kernel<1, false>), dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
+        else if (a.trace)
+            hipLaunchKernelGGL((transport_kernel<0, true>), dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
+        else
+            hipLaunchKernelGGL((transport_kernel<0, false>), dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {

@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--lds-nodes", type=int, default=-1)
     ap.add_argument("--safety-cells", type=int, default=-1)
+    ap.add_argument("--opt", action="append", default=[], help="tuning option key=value (lp_set_option)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -98,6 +99,9 @@ def main():
     ctx = lp.Context(0)
     setup(ctx, sc, seed=0x5EED0001)
     ctx.set_launch(args.grid, args.lds_nodes, args.safety_cells)
+    for kv in args.opt:
+        k, v = kv.split("=")
+        ctx.set_option(k, float(v))
     # lp_run blocks on a HIP event recorded after the kernel on the context's
     # stream, so the timed region is bracketed by completed device work.  torch
     # is not initialised on the GPU here: it ships its own HIP/HSA runtime, and
@@ -156,8 +160,9 @@ def main():
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "work": "algorithmic fp32 per step: interaction %g + boundary %g, search %g (T=%d)" %
                              (F_INT, F_BND, search, T)},
-        "counters": {k: m[k] for k in ("interact", "boundary", "escaped", "capped", "node_visits", "tri_tests",
-                                       "skipped")},
+        "counters": {k: m[k] for k in ("interact", "boundary", "escaped", "capped", "nan", "node_visits",
+                                       "tri_tests", "skipped", "dist_queries", "dist_visits")},
+        "options": args.opt,
         "physics": {"R_total": float(R[0]), "F_total": float(R[1]), "T_total": float(R[2]),
                     "detector_R": [float(x) for x in det]},
         "grid_blocks": st.grid_blocks, "bvh_nodes": st.bvh_nodes, "bvh_depth": st.bvh_depth,

@@ -163,6 +163,18 @@ int lp_debug_math(lp_context *ctx, int32_t op, const float *xy, float *out, int6
  * off, -1 = automatic). */
 int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32_t safety_cells);
 
+/* Tuning options (no effect on results, only on speed). */
+#define LP_OPT_GRID_BLOCKS 0    /* persistent work-groups, 0 = automatic            */
+#define LP_OPT_LDS_NODES 1      /* BVH nodes staged in LDS (0..2048)                */
+#define LP_OPT_SAFETY_CELLS 2   /* safety grid cells on the longest axis (0 = off)  */
+#define LP_OPT_DIST_CAP 3       /* free-radius query: search cap in length units (0 = off) */
+#define LP_OPT_DIST_CAP_STEPS 4 /* free-radius query: cap in units of the step length */
+#define LP_OPT_BLOCKS_PER_CU 5  /* work-groups per CU when GRID_BLOCKS is automatic */
+#define LP_OPT_FREE_ITERS 6     /* search-free steps a lane may take per round      */
+#define LP_OPT_WAVES_PER_SIMD 7 /* register budget of the kernel variant: 1 (compiler's
+                                   choice), 4, 5 or 6 waves per SIMD               */
+int lp_set_option(lp_context *ctx, int32_t key, double value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,8 @@ RNG_PHILOX, RNG_RAN3 = 0, 1
 TALLY = {"sft": 0, "rtd": 1, "arz": 2, "phirz": 3, "region": 4, "vox": 5, "rad": 6, "det": 7, "misc": 8}
 MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out",
         "node_visits", "tri_tests", "skipped", "nan", "dist_queries", "dist_visits"]
+OPTIONS = {"grid_blocks": 0, "lds_nodes": 1, "safety_cells": 2, "dist_cap": 3, "dist_cap_steps": 4,
+           "blocks_per_cu": 5, "free_iters": 6, "waves_per_simd": 7}
 FIXED = 2.0 ** 32
 FIXED_PHI = 2.0 ** 24
 
@@ -63,6 +65,7 @@ _SIGS = {
     "lp_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "lp_debug_math": (C.c_int, [C.c_void_p, C.c_int32, _fp, _fp, C.c_int64]),
     "lp_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "lp_set_option": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
     "lpm_mf_default": (None, [C.c_void_p]),
     "lpm_mf_num_triangles": (C.c_int64, [C.c_void_p]),
     "lpm_mf_build": (C.c_int, [C.c_void_p, _fp, _fp, _ip, C.c_int64]),
@@ -168,6 +171,10 @@ class Context:
 
     def set_launch(self, grid_blocks=0, lds_nodes=-1, safety_cells=-1):
         self._chk(lib().lp_set_launch(self._c, grid_blocks, lds_nodes, safety_cells), "lp_set_launch")
+
+    def set_option(self, key, value):
+        k = OPTIONS[key] if isinstance(key, str) else int(key)
+        self._chk(lib().lp_set_option(self._c, k, float(value)), "lp_set_option(%s)" % key)
 
     def set_scene(self, sc, seed=1, rng=RNG_PHILOX, max_steps=0):
         """Configure from a scene dict (tri, nrm, med, nmuasg, fetal_id, pstart,

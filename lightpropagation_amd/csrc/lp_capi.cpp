@@ -56,7 +56,10 @@ struct lp_context {
     uint32_t tally_mask = 0xffffffffu;
     int32_t record_exit = 0;
     int32_t grid_override = 0, lds_override = -1, safety_cells = -1;
-    float dist_cap = 1.0f, dist_cap_steps = 64.0f;
+    float dist_cap = 0.2f, dist_cap_steps = 16.0f;
+    int32_t blocks_per_cu = 4;
+    int32_t free_iters = 16;
+    int32_t waves = 1;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
 
     // derived
@@ -213,7 +216,7 @@ int prepare(lp_context* c)
         c->bvh = build_bvh(c->tri.data(), c->med.data(), c->ntri, 4);
         c->bvh_extent = 0.0f;
         for (float x : c->tri) c->bvh_extent = std::max(c->bvh_extent, std::fabs(x));
-        int cells = c->safety_cells < 0 ? 128 : c->safety_cells;
+        int cells = c->safety_cells < 0 ? 0 : c->safety_cells;  // off unless asked: the free radius covers it
         c->sg = cells > 0 ? build_safety_grid(c->bvh, c->tri.data(), c->ntri, cells) : SafetyGrid();
         if ((rc = upload(c, c->d_tris, c->bvh.tris.data(), c->bvh.tris.size() * sizeof(TriRec)))) return rc;
         if ((rc = upload(c, c->d_nodes, c->bvh.nodes.data(), c->bvh.nodes.size() * sizeof(BvhNode)))) return rc;
@@ -255,6 +258,8 @@ void fill_args(lp_context* c, KArgs& a)
     a.ntri = (int32_t)c->ntri;
     a.dist_cap = c->dist_cap;
     a.dist_cap_steps = c->dist_cap_steps;
+    a.free_iters = c->free_iters;
+    a.waves = c->waves;
     a.safe_margin = 2e-6f * c->bvh_extent;
     if (!c->sg.dist.empty()) {
         a.safety = (const float*)c->d_safety.p;
@@ -324,8 +329,7 @@ int launch(lp_context* c, KArgs& a)
     int grid = c->grid_override;
     if (grid <= 0) {
         // enough work-groups to fill every CU at the occupancy the kernel allows
-        int per_cu = 4;
-        grid = std::max(1, c->num_cus) * per_cu;
+        grid = std::max(1, c->num_cus) * c->blocks_per_cu;
     }
     int64_t max_grid = (a.count + kBlock - 1) / kBlock;
     grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, std::max<int64_t>(max_grid, 1)));
@@ -511,6 +515,48 @@ int lp_set_launch(lp_context* c, int32_t grid_blocks, int32_t lds_nodes, int32_t
     if (safety_cells != c->safety_cells) c->mesh_dirty = true;
     c->safety_cells = safety_cells;
     return LP_OK;
+}
+
+int lp_set_option(lp_context* c, int32_t key, double v)
+{
+    if (!c) return LP_EINVAL;
+    switch (key) {
+    case LP_OPT_GRID_BLOCKS:
+        if (v < 0 || v > 1e7) return fail(c, LP_EINVAL, "grid blocks out of range");
+        c->grid_override = (int32_t)v;
+        return LP_OK;
+    case LP_OPT_LDS_NODES:
+        if (v < -1 || v > 2048) return fail(c, LP_EINVAL, "LDS nodes out of range");
+        c->lds_override = (int32_t)v;
+        return LP_OK;
+    case LP_OPT_SAFETY_CELLS:
+        if (v < -1 || v > 1024) return fail(c, LP_EINVAL, "safety cells out of range");
+        if ((int32_t)v != c->safety_cells) c->mesh_dirty = true;
+        c->safety_cells = (int32_t)v;
+        return LP_OK;
+    case LP_OPT_DIST_CAP:
+        if (!(v >= 0)) return fail(c, LP_EINVAL, "distance cap must be >= 0");
+        c->dist_cap = (float)v;
+        return LP_OK;
+    case LP_OPT_DIST_CAP_STEPS:
+        if (!(v >= 0)) return fail(c, LP_EINVAL, "distance cap steps must be >= 0");
+        c->dist_cap_steps = (float)v;
+        return LP_OK;
+    case LP_OPT_FREE_ITERS:
+        if (v < 1 || v > 4096) return fail(c, LP_EINVAL, "free iterations out of range");
+        c->free_iters = (int32_t)v;
+        return LP_OK;
+    case LP_OPT_WAVES_PER_SIMD:
+        if (!(v == 1 || v == 4 || v == 5 || v == 6)) return fail(c, LP_EINVAL, "waves per SIMD must be 1, 4, 5 or 6");
+        c->waves = (int32_t)v;
+        return LP_OK;
+    case LP_OPT_BLOCKS_PER_CU:
+        if (v < 1 || v > 32) return fail(c, LP_EINVAL, "blocks per CU out of range");
+        c->blocks_per_cu = (int32_t)v;
+        return LP_OK;
+    default:
+        return fail(c, LP_EINVAL, "unknown option %d", key);
+    }
 }
 
 int lp_get_layout(lp_context* c, lp_layout* out)

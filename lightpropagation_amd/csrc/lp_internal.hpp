@@ -76,6 +76,8 @@ struct KArgs {
     float dist_cap;        // per-photon free-radius search: cap (0 = off)
     float dist_cap_steps;  // ... and cap in units of the current step length
     float safe_margin;     // absolute rounding margin of the free radius
+    int32_t free_iters;    // search-free steps a lane may take per round
+    int32_t waves;         // occupancy target of the kernel variant (1, 4, 5, 6)
     float nmuasg[4 * kMaxTissues];
     int32_t ntissue, fetal_id;
     float pstart[3];

@@ -36,9 +36,10 @@ struct Ran3Lds {
 // random numbers
 // ---------------------------------------------------------------------------
 struct PhiloxState {
-    uint32_t k0, k1, id0, id1, block;  // k0, k1: the (uniform) seed
-    uint32_t buf[4];
-    int used;
+    uint32_t id0, id1, block;  // photon index and the next counter block
+    uint32_t b0, b1, b2, b3;   // unused words of the current block, b0 first
+    int left;                  // how many of b0..b3 are unused
+    uint32_t k0, k1;           // the seed
     __device__ void init(uint64_t seed, uint64_t photon)
     {
         k0 = (uint32_t)seed;
@@ -46,28 +47,26 @@ struct PhiloxState {
         id0 = (uint32_t)photon;
         id1 = (uint32_t)(photon >> 32);
         block = 0;
-        used = 4;
+        left = 0;
     }
     __device__ __forceinline__ float next()
     {
-        if (used == 4) {
+        if (left == 0) {
             uint32_t c[4] = {block, id0, id1, 0u};
             philox10(c, k0, k1);
-            buf[0] = c[0];
-            buf[1] = c[1];
-            buf[2] = c[2];
-            buf[3] = c[3];
+            b0 = c[0];
+            b1 = c[1];
+            b2 = c[2];
+            b3 = c[3];
             block++;
-            used = 0;
+            left = 4;
         }
-        uint32_t u;
-        switch (used) {  // keeps buf[] in registers
-        case 0: u = buf[0]; break;
-        case 1: u = buf[1]; break;
-        case 2: u = buf[2]; break;
-        default: u = buf[3]; break;
-        }
-        used++;
+        // take b0 and shift: no runtime-indexed array (it would live in scratch)
+        uint32_t u = b0;
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        left--;
         return (float)(u >> 8) * 5.96046448e-08f;
     }
 };
@@ -329,8 +328,10 @@ struct Photon {
     int64_t k;  // index within this run
 };
 
-template <int RNG, bool TRACE>
-__global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
+// WAVES: minimum waves per SIMD the register allocator must allow (1 =
+// the compiler's choice); a tuning knob, results do not depend on it.
+template <int RNG, bool TRACE, int WAVES>
+__global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
 {
     extern __shared__ float4 smem[];
     float4* lnodes = smem;                         // 2 * nlds
@@ -372,290 +373,196 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
     int64_t vox_cell = -1;           // voxel whose deposits are pending in vox_acc
     unsigned long long vox_acc = 0;
 
-    for (;;) {
-        // ---- refill lanes whose photon has finished (one atomic per wave) ----
-        if (!exhausted) {
-            unsigned long long need = __ballot(!alive);
-            if (need) {
-                int nneed = __popcll(need);
-                int leader = __ffsll((long long)need) - 1;
-                unsigned long long got = 0;
-                if (lane == leader) got = atomicAdd(a.counter, (unsigned long long)nneed);
-                uint32_t blo = __shfl((uint32_t)got, leader), bhi = __shfl((uint32_t)(got >> 32), leader);
-                unsigned long long base = ((unsigned long long)bhi << 32) | blo;
-                int rank = __popcll(need & ((1ull << lane) - 1ull));
-                if (base + (unsigned long long)nneed >= (unsigned long long)a.count) exhausted = true;
-                if (!alive) {
-                    unsigned long long k = base + (unsigned long long)rank;
-                    if (k < (unsigned long long)a.count) {
-                        int64_t id = a.first + (int64_t)k;
-                        rng.init(a.seed, (uint64_t)id);
-                        // GetvStart (.cl:423-467): A == 1 -> vStart[0]; A > 1 by cumulative counts
-                        int dsel = 0;
-                        if (a.nstart > 1) {
-                            int64_t sum = 0;
-                            for (int i = 0; i < a.nstart; ++i) {
-                                sum += a.vocc[i];
-                                dsel = i;
-                                if (id < sum) break;
-                            }
-                        }
-                        // PhotonClass::Init (.cl:344-389)
-                        ph.v = mk(a.vstart[3 * dsel], a.vstart[3 * dsel + 1], a.vstart[3 * dsel + 2]);
-                        ph.p = mk(a.pstart[0], a.pstart[1], a.pstart[2]);
-                        ph.w = 1.0f - a.rsp;
-                        ph.sft = mk(0.0f, 0.0f, 0.0f);
-                        ph.unfin = 0.0f;
-                        ph.absorbed = 0.0f;
-                        ph.fetal = 0;
-                        ph.ref = -1;
-                        ph.curr = 0;
-                        ph.nv = mk(0.0f, 0.0f, -1.0f);
-                        ph.roulette = 1;
-                        ph.steps = 0;
-                        ph.safe = 0.0f;
-                        ph.k = (int64_t)k;
-                        reg_acc = 0;
-                        alive = true;
-                        cnt[kPhot]++;
-                    }
-                }
-            }
-        }
-        if (!__any(alive)) break;
-        if (!alive) continue;
+    float stepsize = 0.0f;  // drawn step length, pending until used
+    bool ballistic = false;
+    bool have_step = false;  // a step length has been drawn and not yet used
+    bool search = false;     // the pending step needs a face search
 
-        // ---- one iteration of the loop body (.cl:1375-1575) ----
-        const float4 cm = lmat[ph.curr];  // (n, mua, mus, g)
-        float stepsize;
-        bool ballistic = (cm.y == 0.0f && cm.z == 0.0f);
-        if (!ballistic) {
-            if (ph.unfin == 0.0f) {
-                float x;
-                do {
-                    x = rng.next();
-                } while (x <= 0.0f);
-                stepsize = -logf_lp(x) * a.factor / (cm.y + cm.z);
-            } else {
-                stepsize = ph.unfin / (cm.y + cm.z);
-                ph.unfin = 0.0f;
-            }
+    // interaction branch (.cl:1513-1537)
+    auto interaction = [&](float s) {
+        const float4 cm = lmat[ph.curr];
+        ph.p = add(ph.p, mul(s, ph.v));
+        ph.ref = -1;
+        ph.safe = ph.safe - s;
+        if (cm.y > 0.0f && cm.z > 0.0f) {  // Absorb (.cl:841-847)
+            ph.absorbed = ph.w * (cm.y / (cm.y + cm.z));
+            ph.w -= ph.absorbed;
         }
-        Hit h;
-        bool skip = false;
-        if (!ballistic) {
-            if (stepsize < ph.safe || stepsize < safety_at(a, ph.p)) {
-                skip = true;
-            } else if (a.dist_cap > 0.0f && ph.ref < 0 && __builtin_isfinite(ph.p.x) &&
-                       __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z)) {
-                // refresh the photon's free radius: nearest face distance less
-                // a rounding margin (a face found by the exact test at t <= s
-                // lies within s + margin of p)
-                float cap = fminf(a.dist_cap, a.dist_cap_steps * stepsize);
-                float d = mesh_distance(a, lnodes, ph.p, cap, cnt[kDvis]);
-                float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)), __builtin_fabsf(ph.p.z));
-                ph.safe = d - (a.safe_margin + 1e-5f * (d + rad));
-                cnt[kDq]++;
-                skip = stepsize < ph.safe;
-            }
-        }
-        if (skip) {
-            h.id = -1;
-            cnt[kSkip]++;
-        } else {
-            find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, cnt[kVis], cnt[kTst]);
-        }
-        float stb = 10000.0f;
-        bool boundary;
-        if (h.id >= 0) {
-            stb = h.t;
-            boundary = true;
-        } else {
-            boundary = ballistic || stepsize >= 10000.0f;  // no face ahead at all
-        }
-        if (ballistic) stepsize = stb;
-        int kind;
-        if (boundary) {
-            ph.p = add(ph.p, mul(stb, ph.v));  // MoveStep
-            ph.ref = -1;
-            ph.safe = 0.0f;
-            if (h.id < 0) {  // nothing ahead: the photon has left the mesh
-                kind = 3;
-                ph.w = 0.0f;
-                cnt[kEsc]++;
-            } else {
-                float x = rng.next();
-                int32_t in_id = h.med & 0xffff, out_id = (h.med >> 16) & 0xffff;
-                ph.nv = normalize(mk(a.nrm[3 * h.id], a.nrm[3 * h.id + 1], a.nrm[3 * h.id + 2]));
-                int32_t nx_id = (ph.curr == out_id) ? in_id : out_id;
-                const float4 nm = lmat[nx_id];
-                if (__builtin_fabsf(__builtin_sqrtf(ph.p.x * ph.p.x + ph.p.y * ph.p.y)) < 0.01f)
-                    ph.nv = mul(sign_cl(dot(ph.nv, eZr)), eZ);
-                // TransmitOrReflect (.cl:921-1018)
-                float cos_in = dot(ph.nv, ph.v);
-                float abs_cos = __builtin_fabsf(cos_in);
-                float n1 = cm.x, n2 = nm.x;
-                float cos_crit = 0.0f;
-                if (n1 > n2) {
-                    float q = n2 / n1;
-                    cos_crit = __builtin_sqrtf(1.0f - q * q);
-                }
-                float r;
-                if (abs_cos <= cos_crit) {
-                    r = 1.0f;
-                } else if (n1 == n2) {  // GetFresnel (.cl:857-907)
-                    r = 0.0f;
-                } else if (abs_cos >= 1.0f) {  // normal incidence (> 1 only by rounding)
-                    r = (n1 - n2) / (n1 + n2);
-                    r *= r;
-                } else if (cos_in == 0.0f) {
-                    r = 1.0f;
-                } else {
-                    float sin_in = __builtin_sqrtf(clamp0(1 - cos_in * cos_in));
-                    float sin_out = n1 * sin_in / n2;
-                    if (sin_out >= 1.0f) {
-                        r = 1.0f;
-                    } else {
-                        float cos_out = __builtin_sqrtf(1 - sin_out * sin_out);
-                        float cap = cos_in * cos_out - sin_in * sin_out;
-                        float cam = cos_in * cos_out + sin_in * sin_out;
-                        float sap = sin_in * cos_out + cos_in * sin_out;
-                        float sam = sin_in * cos_out - cos_in * sin_out;
-                        r = 0.5f * (sam * sam) * ((cam * cam) + (cap * cap)) / ((sap * sap) * (cam * cam));
-                    }
-                }
-                f3 ey = normalize(cross(ph.nv, cross(ph.nv, ph.v)));
-                int32_t new_id;
-                float m_a, m_s;
-                if (x > r) {  // Transmit (.cl:506-529)
-                    float sin_t = __builtin_sqrtf(clamp0(1 - cos_in * cos_in)) * n1 / n2;
-                    float cos_t = -__builtin_sqrtf(clamp0(1 - sin_t * sin_t));
-                    if (cos_in > 0.0f) cos_t = __builtin_fabsf(cos_t);
-                    ph.v = normalize(sub(mul(cos_t, ph.nv), mul(sin_t, ey)));
-                    new_id = (cos_in > 0) ? out_id : in_id;
-                    m_a = nm.y;
-                    m_s = nm.z;
-                    kind = 1;
-                } else {  // Reflect (.cl:532-549)
-                    float sin_r = __builtin_sqrtf(clamp0(1 - cos_in * cos_in));
-                    ph.v = neg(normalize(add(mul(cos_in, ph.nv), mul(sin_r, ey))));
-                    new_id = ph.curr;
-                    m_a = cm.y;
-                    m_s = cm.z;
-                    kind = 2;
-                }
-                if (m_a == 0 || m_s == 0) {  // leaves the tissue: record and kill
-                    ph.sft.y = 0;
-                    float dist = __builtin_fabsf(dot(eZ, sub(ph.p, POI)));
-                    float wr = ph.w * (1 - a.rsp);
-                    if (dist > a.dist_min) {
-                        ph.sft.z = wr;
-                    } else {
-                        if (ph.fetal) ph.sft.y = wr;
-                        ph.sft.x = wr;
-                    }
-                    ph.w = 0.0f;
-                }
-                if (new_id != ph.curr) {
-                    if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
-                    reg_acc = 0;
-                }
-                ph.curr = new_id;
-                ph.ref = h.id;
-                if (ph.curr == a.fetal_id) ph.fetal = 1;
-                ph.unfin = (stepsize - stb) * (cm.y + cm.z);
-            }
-            cnt[kBnd]++;
-        } else {
-            // interaction branch (.cl:1513-1537)
-            ph.p = add(ph.p, mul(stepsize, ph.v));
-            ph.ref = -1;
-            ph.safe = ph.safe - stepsize;
-            if (cm.y > 0.0f && cm.z > 0.0f) {  // Absorb (.cl:841-847)
-                ph.absorbed = ph.w * (cm.y / (cm.y + cm.z));
-                ph.w -= ph.absorbed;
-            }
-            float xt = rng.next();
-            float xp = rng.next();
+        float xt = rng.next();
+        float xp = rng.next();
             {  // VScatter (.cl:597-655)
-                const f3 v = ph.v;
-                const float g = cm.w;
-                f3 e_rot = mk(0.0f, 0.0f, 0.0f);
-                if (v.z != 0.0f) {
-                    float z = -(v.x + v.y) / v.z;
-                    e_rot = mk(1.0f, 1.0f, z);
-                } else if (v.y != 0.0f) {
-                    float y = -(v.x) / v.y;
-                    e_rot = mk(1.0f, y, 0.0f);
-                } else if (v.x != 0.0f) {
-                    e_rot = mk(0.0f, 1.0f, 0.0f);
-                }
-                e_rot = normalize(e_rot);
-                float cos_t;
-                float g1 = (1 - g * g) / (1 - g + 2 * g * xt);
-                if (g != 0.0f) {
-                    cos_t = (1 + g * g - g1 * g1) / (2 * g);
-                    if (cos_t < -1) cos_t = -1;
-                    else if (cos_t > 1) cos_t = 1;
-                } else {
-                    cos_t = 2 * xt - 1;
-                }
-                float sin_t = __builtin_sqrtf(1.0f - cos_t * cos_t);
-                float psi = 6.28318548f * xp;
-                float cos_p = cosf_lp(psi);
-                float sin_p = __builtin_sqrtf(1.0f - cos_p * cos_p);
-                if (psi > 3.14159274f) sin_p *= -1;
-                // Rotate (.cl:697-708) twice
-                float d1 = dot(e_rot, v);
-                f3 c1 = cross(e_rot, v);
-                f3 c1n = cross(c1, e_rot);
-                f3 rt = normalize(add(add(mul(d1, e_rot), mul(cos_t, c1n)), mul(sin_t, c1)));
-                float d2 = dot(v, rt);
-                f3 c2 = cross(v, rt);
-                f3 c2n = cross(c2, v);
-                ph.v = normalize(add(add(mul(d2, v), mul(cos_p, c2n)), mul(sin_p, c2)));
+            const f3 v = ph.v;
+            const float g = cm.w;
+            f3 e_rot = mk(0.0f, 0.0f, 0.0f);
+            if (v.z != 0.0f) {
+                float z = -(v.x + v.y) / v.z;
+                e_rot = mk(1.0f, 1.0f, z);
+            } else if (v.y != 0.0f) {
+                float y = -(v.x) / v.y;
+                e_rot = mk(1.0f, y, 0.0f);
+            } else if (v.x != 0.0f) {
+                e_rot = mk(0.0f, 1.0f, 0.0f);
             }
-            // absorption tallies at the new position (replaces RecordAbsorbtionRZ, .cl:254-271)
-            const f3 d = sub(ph.p, POI);
-            const f3 c = mk(dot(eX, d), dot(eY, d), dot(eZ, d));
-            unsigned long long q = fx32(ph.absorbed);
-            reg_acc += q;
-            if (a.arz) {
-                float rho = __builtin_sqrtf(c.x * c.x + c.y * c.y) * sign_cl(c.x) / a.det_grid[3];
-                int32_t ir = cvt_i32f(rho);
-                int32_t iz = cvt_i32f(c.z / a.det_grid[2]);
-                if (ir >= -a.dimR && ir <= a.dimR && iz >= 0 && iz <= a.dimz) {
-                    int64_t cell = (int64_t)iz * a.arz_nr + (ir + a.dimR);
-                    atomic_add_u64(&a.arz[cell], q);
-                    atomic_add_u64(&a.phirz[cell], fx24(ph.absorbed / (cm.y == 0.0f ? 1.0f : cm.y)));
-                } else {
-                    c_arz_out += q;
-                }
+            e_rot = normalize(e_rot);
+            float cos_t;
+            float g1 = (1 - g * g) / (1 - g + 2 * g * xt);
+            if (g != 0.0f) {
+                cos_t = (1 + g * g - g1 * g1) / (2 * g);
+                if (cos_t < -1) cos_t = -1;
+                else if (cos_t > 1) cos_t = 1;
+            } else {
+                cos_t = 2 * xt - 1;
             }
-            if (a.vox) {
-                float fx = (c.x - a.vox_origin[0]) / a.vox_size[0];
-                float fy = (c.y - a.vox_origin[1]) / a.vox_size[1];
-                float fz = (c.z - a.vox_origin[2]) / a.vox_size[2];
-                bool ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f;
-                int32_t ix = cvt_i32f(fx), iy = cvt_i32f(fy), iz = cvt_i32f(fz);
-                ok = ok && ix >= 0 && iy >= 0 && iz >= 0 && ix < a.vox_n[0] && iy < a.vox_n[1] && iz < a.vox_n[2];
-                if (ok) {
-                    // consecutive deposits in one voxel are summed in a register
-                    int64_t cell = ((int64_t)iz * a.vox_n[1] + iy) * a.vox_n[0] + ix;
-                    if (cell != vox_cell) {
-                        if (vox_cell >= 0) atomic_add_u64(&a.vox[vox_cell], vox_acc);
-                        vox_cell = cell;
-                        vox_acc = 0;
-                    }
-                    vox_acc += q;
-                } else {
-                    c_vox_out += q;
-                }
-            }
-            cnt[kInt]++;
-            kind = 0;
+            float sin_t = __builtin_sqrtf(1.0f - cos_t * cos_t);
+            float psi = 6.28318548f * xp;
+            float cos_p = cosf_lp(psi);
+            float sin_p = __builtin_sqrtf(1.0f - cos_p * cos_p);
+            if (psi > 3.14159274f) sin_p *= -1;
+            // Rotate (.cl:697-708) twice
+            float d1 = dot(e_rot, v);
+            f3 c1 = cross(e_rot, v);
+            f3 c1n = cross(c1, e_rot);
+            f3 rt = normalize(add(add(mul(d1, e_rot), mul(cos_t, c1n)), mul(sin_t, c1)));
+            float d2 = dot(v, rt);
+            f3 c2 = cross(v, rt);
+            f3 c2n = cross(c2, v);
+            ph.v = normalize(add(add(mul(d2, v), mul(cos_p, c2n)), mul(sin_p, c2)));
         }
-        // Roulette (.cl:1020-1035, 1539-1545)
+            // absorption tallies at the new position (replaces RecordAbsorbtionRZ, .cl:254-271)
+        const f3 d = sub(ph.p, POI);
+        const f3 c = mk(dot(eX, d), dot(eY, d), dot(eZ, d));
+        unsigned long long q = fx32(ph.absorbed);
+        reg_acc += q;
+        if (a.arz) {
+            float rho = __builtin_sqrtf(c.x * c.x + c.y * c.y) * sign_cl(c.x) / a.det_grid[3];
+            int32_t ir = cvt_i32f(rho);
+            int32_t iz = cvt_i32f(c.z / a.det_grid[2]);
+            if (ir >= -a.dimR && ir <= a.dimR && iz >= 0 && iz <= a.dimz) {
+                int64_t cell = (int64_t)iz * a.arz_nr + (ir + a.dimR);
+                atomic_add_u64(&a.arz[cell], q);
+                atomic_add_u64(&a.phirz[cell], fx24(ph.absorbed / (cm.y == 0.0f ? 1.0f : cm.y)));
+            } else {
+                c_arz_out += q;
+            }
+        }
+        if (a.vox) {
+            float fx = (c.x - a.vox_origin[0]) / a.vox_size[0];
+            float fy = (c.y - a.vox_origin[1]) / a.vox_size[1];
+            float fz = (c.z - a.vox_origin[2]) / a.vox_size[2];
+            bool ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f;
+            int32_t ix = cvt_i32f(fx), iy = cvt_i32f(fy), iz = cvt_i32f(fz);
+            ok = ok && ix >= 0 && iy >= 0 && iz >= 0 && ix < a.vox_n[0] && iy < a.vox_n[1] && iz < a.vox_n[2];
+            // consecutive deposits in one voxel are summed in a register
+            // (value selects, so the accumulator stays in registers)
+            int64_t cell = ((int64_t)iz * a.vox_n[1] + iy) * a.vox_n[0] + ix;
+            bool same = ok && cell == vox_cell;
+            if (ok && !same && vox_cell >= 0) atomic_add_u64(&a.vox[vox_cell], vox_acc);
+            vox_acc = same ? vox_acc + q : (ok ? q : vox_acc);
+            vox_cell = ok ? cell : vox_cell;
+            c_vox_out += ok ? 0ull : (unsigned long long)q;
+        }
+        cnt[kInt]++;
+    };
+
+    // boundary branch (.cl:1462-1511) at face h.id, distance stb
+    auto boundary_event = [&](const Hit& h, float s, float stb) -> int {
+        const float4 cm = lmat[ph.curr];
+        int kind;
+        ph.p = add(ph.p, mul(stb, ph.v));  // MoveStep
+        ph.ref = -1;
+        ph.safe = 0.0f;
+        if (h.id < 0) {  // nothing ahead: the photon has left the mesh
+            kind = 3;
+            ph.w = 0.0f;
+            cnt[kEsc]++;
+        } else {
+            float x = rng.next();
+            int32_t in_id = h.med & 0xffff, out_id = (h.med >> 16) & 0xffff;
+            ph.nv = normalize(mk(a.nrm[3 * h.id], a.nrm[3 * h.id + 1], a.nrm[3 * h.id + 2]));
+            int32_t nx_id = (ph.curr == out_id) ? in_id : out_id;
+            const float4 nm = lmat[nx_id];
+            if (__builtin_fabsf(__builtin_sqrtf(ph.p.x * ph.p.x + ph.p.y * ph.p.y)) < 0.01f)
+                ph.nv = mul(sign_cl(dot(ph.nv, eZr)), eZ);
+            // TransmitOrReflect (.cl:921-1018)
+            float cos_in = dot(ph.nv, ph.v);
+            float abs_cos = __builtin_fabsf(cos_in);
+            float n1 = cm.x, n2 = nm.x;
+            float cos_crit = 0.0f;
+            if (n1 > n2) {
+                float q = n2 / n1;
+                cos_crit = __builtin_sqrtf(1.0f - q * q);
+            }
+            float r;
+            if (abs_cos <= cos_crit) {
+                r = 1.0f;
+            } else if (n1 == n2) {  // GetFresnel (.cl:857-907)
+                r = 0.0f;
+            } else if (abs_cos >= 1.0f) {  // normal incidence (> 1 only by rounding)
+                r = (n1 - n2) / (n1 + n2);
+                r *= r;
+            } else if (cos_in == 0.0f) {
+                r = 1.0f;
+            } else {
+                float sin_in = __builtin_sqrtf(clamp0(1 - cos_in * cos_in));
+                float sin_out = n1 * sin_in / n2;
+                if (sin_out >= 1.0f) {
+                    r = 1.0f;
+                } else {
+                    float cos_out = __builtin_sqrtf(1 - sin_out * sin_out);
+                    float cap = cos_in * cos_out - sin_in * sin_out;
+                    float cam = cos_in * cos_out + sin_in * sin_out;
+                    float sap = sin_in * cos_out + cos_in * sin_out;
+                    float sam = sin_in * cos_out - cos_in * sin_out;
+                    r = 0.5f * (sam * sam) * ((cam * cam) + (cap * cap)) / ((sap * sap) * (cam * cam));
+                }
+            }
+            f3 ey = normalize(cross(ph.nv, cross(ph.nv, ph.v)));
+            int32_t new_id;
+            float m_a, m_s;
+            if (x > r) {  // Transmit (.cl:506-529)
+                float sin_t = __builtin_sqrtf(clamp0(1 - cos_in * cos_in)) * n1 / n2;
+                float cos_t = -__builtin_sqrtf(clamp0(1 - sin_t * sin_t));
+                if (cos_in > 0.0f) cos_t = __builtin_fabsf(cos_t);
+                ph.v = normalize(sub(mul(cos_t, ph.nv), mul(sin_t, ey)));
+                new_id = (cos_in > 0) ? out_id : in_id;
+                m_a = nm.y;
+                m_s = nm.z;
+                kind = 1;
+            } else {  // Reflect (.cl:532-549)
+                float sin_r = __builtin_sqrtf(clamp0(1 - cos_in * cos_in));
+                ph.v = neg(normalize(add(mul(cos_in, ph.nv), mul(sin_r, ey))));
+                new_id = ph.curr;
+                m_a = cm.y;
+                m_s = cm.z;
+                kind = 2;
+            }
+            if (m_a == 0 || m_s == 0) {  // leaves the tissue: record and kill
+                // transmitted past distMin, else reflected (and fetal-tagged);
+                // written as value selects so that sft stays in registers
+                float dist = __builtin_fabsf(dot(eZ, sub(ph.p, POI)));
+                float wr = ph.w * (1 - a.rsp);
+                bool trans = dist > a.dist_min;
+                ph.sft.z = trans ? wr : ph.sft.z;
+                ph.sft.x = trans ? ph.sft.x : wr;
+                ph.sft.y = (!trans && ph.fetal) ? wr : 0.0f;
+                ph.w = 0.0f;
+            }
+            if (new_id != ph.curr) {
+                if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
+                reg_acc = 0;
+            }
+            ph.curr = new_id;
+            ph.ref = h.id;
+            if (ph.curr == a.fetal_id) ph.fetal = 1;
+            ph.unfin = (s - stb) * (cm.y + cm.z);
+        }
+        cnt[kBnd]++;
+        return kind;
+    };
+
+    // Roulette (.cl:1020-1035, 1539-1545), trace, step cap; false when the
+    // photon has ended
+    auto end_step = [&](int kind, int hid, float s) -> bool {
         if (ph.w < a.w_th && ph.w > 0.0f) {
             float x = rng.next();
             if (x < a.chance) {
@@ -671,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
         if (TRACE && ph.steps <= a.trace_max) {
             float* t = a.trace + ((int64_t)ph.k * a.trace_max + (ph.steps - 1)) * 12;
             t[0] = (float)kind;
-            t[1] = (float)(boundary ? h.id : -1);
+            t[1] = (float)hid;
             t[2] = (float)ph.curr;
             t[3] = ph.w;
             t[4] = ph.p.x;
@@ -680,17 +587,18 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
             t[7] = ph.v.x;
             t[8] = ph.v.y;
             t[9] = ph.v.z;
-            t[10] = stepsize;
+            t[10] = s;
             t[11] = (float)(TRACE ? rng.ndraw : 0);
         }
         if (a.max_steps > 0 && ph.steps >= a.max_steps && ph.w > 0.0f) {
             cnt[kCap]++;
             ph.w = 0.0f;
         }
-        if (ph.w > 0.0f && (ph.w > a.w_th || ph.roulette)) continue;
+        return ph.w > 0.0f && (ph.w > a.w_th || ph.roulette);
+    };
 
-        // ---- epilogue (.cl:1586-1594) + host binning (cpp:1553-1630) ----
-        alive = false;
+    // epilogue (.cl:1586-1594) + host binning (cpp:1553-1630)
+    auto epilogue = [&]() {
         if (ph.p.x != ph.p.x || ph.p.y != ph.p.y || ph.p.z != ph.p.z || ph.v.x != ph.v.x || ph.v.y != ph.v.y ||
             ph.v.z != ph.v.z)
             cnt[kNan]++;
@@ -768,6 +676,151 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a)
                 cnt[i] = 0;
             }
         }
+    };
+
+    for (;;) {
+        // ---- refill lanes whose photon has finished (one atomic per wave) ----
+        if (!exhausted) {
+            unsigned long long need = __ballot(!alive);
+            if (need) {
+                int nneed = __popcll(need);
+                int leader = __ffsll((long long)need) - 1;
+                unsigned long long got = 0;
+                if (lane == leader) got = atomicAdd(a.counter, (unsigned long long)nneed);
+                uint32_t blo = __shfl((uint32_t)got, leader), bhi = __shfl((uint32_t)(got >> 32), leader);
+                unsigned long long base = ((unsigned long long)bhi << 32) | blo;
+                int rank = __popcll(need & ((1ull << lane) - 1ull));
+                if (base + (unsigned long long)nneed >= (unsigned long long)a.count) exhausted = true;
+                if (!alive) {
+                    unsigned long long k = base + (unsigned long long)rank;
+                    if (k < (unsigned long long)a.count) {
+                        int64_t id = a.first + (int64_t)k;
+                        rng.init(a.seed, (uint64_t)id);
+                        // GetvStart (.cl:423-467): A == 1 -> vStart[0]; A > 1 by cumulative counts
+                        int dsel = 0;
+                        if (a.nstart > 1) {
+                            int64_t sum = 0;
+                            for (int i = 0; i < a.nstart; ++i) {
+                                sum += a.vocc[i];
+                                dsel = i;
+                                if (id < sum) break;
+                            }
+                        }
+                        // PhotonClass::Init (.cl:344-389)
+                        ph.v = mk(a.vstart[3 * dsel], a.vstart[3 * dsel + 1], a.vstart[3 * dsel + 2]);
+                        ph.p = mk(a.pstart[0], a.pstart[1], a.pstart[2]);
+                        ph.w = 1.0f - a.rsp;
+                        ph.sft = mk(0.0f, 0.0f, 0.0f);
+                        ph.unfin = 0.0f;
+                        ph.absorbed = 0.0f;
+                        ph.fetal = 0;
+                        ph.ref = -1;
+                        ph.curr = 0;
+                        ph.nv = mk(0.0f, 0.0f, -1.0f);
+                        ph.roulette = 1;
+                        ph.steps = 0;
+                        ph.safe = 0.0f;
+                        ph.k = (int64_t)k;
+                        reg_acc = 0;
+                        alive = true;
+                        have_step = false;
+                        search = false;
+                        cnt[kPhot]++;
+                    }
+                }
+            }
+        }
+        if (!__any(alive)) break;
+
+        // ---- one round ----
+        // Each lane draws its step length (.cl:1405-1442) when none is
+        // pending.  A step shorter than the lane's free radius needs no face
+        // search and is taken at once; a lane whose step may reach a face
+        // waits.  The search phases run once per round, for all waiting lanes
+        // together, after up to free_iters search-free iterations (or as soon
+        // as no lane can move without a search).
+        for (int it = 0;; ++it) {
+            bool go = false;  // this lane takes a step in this iteration
+            bool bnd = false;
+            Hit h;
+            h.id = -1;
+            h.t = 10000.0f;
+            bool act = alive && !search;
+            if (act) {
+                if (!have_step) {
+                    const float4 cm = lmat[ph.curr];
+                    ballistic = (cm.y == 0.0f && cm.z == 0.0f);
+                    if (!ballistic) {
+                        if (ph.unfin == 0.0f) {
+                            float x;
+                            do {
+                                x = rng.next();
+                            } while (x <= 0.0f);
+                            stepsize = -logf_lp(x) * a.factor / (cm.y + cm.z);
+                        } else {
+                            stepsize = ph.unfin / (cm.y + cm.z);
+                            ph.unfin = 0.0f;
+                        }
+                    }
+                    have_step = true;
+                }
+                if (!ballistic && (stepsize < ph.safe || (a.safety && stepsize < safety_at(a, ph.p)))) {
+                    go = true;
+                    cnt[kSkip]++;
+                } else {
+                    search = true;
+                }
+            }
+            const bool search_round = it + 1 >= a.free_iters || !__any(go);
+            if (search_round) {
+                // phase B: refresh the free radius of waiting lanes -- the
+                // nearest-face distance less a rounding margin (a face found by
+                // the exact test at t <= s lies within s + margin of p)
+                bool dq = search && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && __builtin_isfinite(ph.p.x) &&
+                          __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
+                if (__any(dq)) {
+                    if (dq) {
+                        float cap = fminf(a.dist_cap, a.dist_cap_steps * stepsize);
+                        float d = mesh_distance(a, lnodes, ph.p, cap, cnt[kDvis]);
+                        float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
+                                          __builtin_fabsf(ph.p.z));
+                        ph.safe = d - (a.safe_margin + 1e-5f * (d + rad));
+                        cnt[kDq]++;
+                        if (stepsize < ph.safe) {
+                            search = false;
+                            go = true;
+                            cnt[kSkip]++;
+                        }
+                    }
+                }
+                // phase C: nearest face within the pending step
+                if (__any(search)) {
+                    if (search) {
+                        find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, cnt[kVis],
+                                  cnt[kTst]);
+                        bnd = h.id >= 0 || ballistic || stepsize >= 10000.0f;  // no face ahead at all: escape
+                        if (h.id < 0) h.t = 10000.0f;
+                        if (ballistic) stepsize = h.t;
+                        search = false;
+                        go = true;
+                    }
+                }
+            }
+            // ---- the step itself (the reference loop body, .cl:1375-1575) ----
+            if (go) {
+                int kind = 0;
+                if (bnd)
+                    kind = boundary_event(h, stepsize, h.t);
+                else
+                    interaction(stepsize);
+                have_step = false;
+                if (!end_step(kind, bnd ? h.id : -1, stepsize)) {
+                    epilogue();
+                    alive = false;
+                }
+            }
+            if (search_round) break;
+        }
     }
     if (a.misc) {
         const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
@@ -796,20 +849,20 @@ int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errl
     hipError_t e;
     const void* fn;
     if (a.rng_mode == 1) lds += sizeof(Ran3Lds);
+    const int w = a.waves;
+#define LP_PICK(R, T, W) (const void*)transport_kernel<R, T, W>
     if (a.rng_mode == 1)
-        fn = a.trace ? (const void*)transport_kernel<1, true> : (const void*)transport_kernel<1, false>;
+        fn = a.trace ? LP_PICK(1, true, 1) : LP_PICK(1, false, 1);
+    else if (a.trace)
+        fn = LP_PICK(0, true, 1);
     else
-        fn = a.trace ? (const void*)transport_kernel<0, true> : (const void*)transport_kernel<0, false>;
+        fn = w == 4 ? LP_PICK(0, false, 4) : w == 5 ? LP_PICK(0, false, 5) : w == 6 ? LP_PICK(0, false, 6)
+                                                                        : LP_PICK(0, false, 1);
+#undef LP_PICK
     e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) {
-        if (a.rng_mode == 1 && a.trace)
-            hipLaunchKernelGGL((transport_kernel<1, true>), dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
-        else if (a.rng_mode == 1)
-            hipLaunchKernelGGL((transport_kernel<1, false>), dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
-        else if (a.trace)
-            hipLaunchKernelGGL((transport_kernel<0, true>), dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
-        else
-            hipLaunchKernelGGL((transport_kernel<0, false>), dim3(grid), dim3(kBlock), lds, (hipStream_t)stream, a);
+        void* args[] = {const_cast<KArgs*>(&a)};
+        e = hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, lds, (hipStream_t)stream);
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {

@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--lds-nodes", type=int, default=-1)
     ap.add_argument("--safety-cells", type=int, default=-1)
     ap.add_argument("--opt", action="append", default=[], help="tuning option key=value (lp_set_option)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run a plain copy of this many MiB (reference for profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -161,12 +163,16 @@ def main():
                      "work": "algorithmic fp32 per step: interaction %g + boundary %g, search %g (T=%d)" %
                              (F_INT, F_BND, search, T)},
         "counters": {k: m[k] for k in ("interact", "boundary", "escaped", "capped", "nan", "node_visits",
-                                       "tri_tests", "skipped", "dist_queries", "dist_visits")},
+                                       "tri_tests", "skipped", "dist_queries", "dist_visits", "list_tests")},
         "options": args.opt,
         "physics": {"R_total": float(R[0]), "F_total": float(R[1]), "T_total": float(R[2]),
                     "detector_R": [float(x) for x in det]},
         "grid_blocks": st.grid_blocks, "bvh_nodes": st.bvh_nodes, "bvh_depth": st.bvh_depth,
     }
+    if args.calibrate_copy:
+        ms = ctx.debug_copy(args.calibrate_copy << 20)
+        out["copy_calibration"] = {"bytes": args.calibrate_copy << 20, "ms": ms,
+                                   "GB_per_s": 2 * (args.calibrate_copy << 20) / ms / 1e6}
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)

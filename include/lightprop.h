@@ -50,7 +50,9 @@ extern "C" {
 #define LP_TALLY_MISC 8   /* counters: photons, steps, interactions, boundary
                              events, escaped, capped, off-grid Arz weight,
                              off-grid voxel weight, BVH node visits, triangle
-                             tests, skipped searches, NaN photons (16 slots)  */
+                             tests, skipped searches, NaN photons, free-radius
+                             queries, their node visits, list face tests
+                             (16 slots)                                       */
 #define LP_NTALLY 9
 
 /* Tally values are int64 fixed point: 2^32 per unit photon weight, except
@@ -158,6 +160,11 @@ int lp_get_stats(lp_context *ctx, lp_stats *out);
  * 1 divide, 2 log, 3 cos, 4 acos, 5 multiply, 6 add) over n (x, y) pairs. */
 int lp_debug_math(lp_context *ctx, int32_t op, const float *xy, float *out, int64_t n);
 
+/* Runs a plain device-to-device copy of `bytes` (multiple of 16) through a
+ * streaming kernel and reports its time: the reference for converting the
+ * profiler's FETCH_SIZE / WRITE_SIZE counters into bytes. */
+int lp_debug_copy(lp_context *ctx, int64_t bytes, double *ms);
+
 /* Launch shape override: persistent work-groups (0 = automatic) and BVH
  * nodes staged in LDS (-1 = automatic), safety grid cells per axis (0 =
  * off, -1 = automatic). */
@@ -173,6 +180,8 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
 #define LP_OPT_FREE_ITERS 6     /* search-free steps a lane may take per round      */
 #define LP_OPT_WAVES_PER_SIMD 7 /* register budget of the kernel variant: 1 (compiler's
                                    choice), 4, 5 or 6 waves per SIMD               */
+#define LP_OPT_NEAR_R0 8        /* near-face lists: radius r0 (0 = off, default 0.1) */
+#define LP_OPT_NEAR_CELL 9      /* near-face lists: cell edge (0 = 2 r0)            */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
 
 #ifdef __cplusplus

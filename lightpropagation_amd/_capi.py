@@ -15,9 +15,9 @@ LP_OK = 0
 RNG_PHILOX, RNG_RAN3 = 0, 1
 TALLY = {"sft": 0, "rtd": 1, "arz": 2, "phirz": 3, "region": 4, "vox": 5, "rad": 6, "det": 7, "misc": 8}
 MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out",
-        "node_visits", "tri_tests", "skipped", "nan", "dist_queries", "dist_visits"]
+        "node_visits", "tri_tests", "skipped", "nan", "dist_queries", "dist_visits", "list_tests"]
 OPTIONS = {"grid_blocks": 0, "lds_nodes": 1, "safety_cells": 2, "dist_cap": 3, "dist_cap_steps": 4,
-           "blocks_per_cu": 5, "free_iters": 6, "waves_per_simd": 7}
+           "blocks_per_cu": 5, "free_iters": 6, "waves_per_simd": 7, "near_r0": 8, "near_cell": 9}
 FIXED = 2.0 ** 32
 FIXED_PHI = 2.0 ** 24
 
@@ -66,6 +66,7 @@ _SIGS = {
     "lp_debug_math": (C.c_int, [C.c_void_p, C.c_int32, _fp, _fp, C.c_int64]),
     "lp_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "lp_set_option": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "lp_debug_copy": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
     "lpm_mf_default": (None, [C.c_void_p]),
     "lpm_mf_num_triangles": (C.c_int64, [C.c_void_p]),
     "lpm_mf_build": (C.c_int, [C.c_void_p, _fp, _fp, _ip, C.c_int64]),
@@ -225,6 +226,11 @@ class Context:
         s = Stats()
         self._chk(lib().lp_get_stats(self._c, C.byref(s)), "lp_get_stats")
         return s
+
+    def debug_copy(self, nbytes):
+        ms = C.c_double()
+        self._chk(lib().lp_debug_copy(self._c, int(nbytes), C.byref(ms)), "lp_debug_copy")
+        return ms.value
 
     def debug_math(self, op, x, y=None):
         x = _f(x)

@@ -316,4 +316,69 @@ SafetyGrid build_safety_grid(const Bvh& bvh, const float* tri, int64_t ntri, int
     return g;
 }
 
+NearGrid build_near_grid(const Bvh& bvh, float r0, float cell)
+{
+    NearGrid g;
+    const int64_t nt = (int64_t)bvh.tris.size();
+    if (nt == 0 || !(r0 > 0) || !(cell > 0)) return g;
+    // boxes of the stored triangles (M, M + E, M + F), widened by r0 + pad
+    const float grow = r0 + bvh.pad;
+    std::vector<float> lo(3 * nt), hi(3 * nt);
+    Box all;
+    for (int64_t i = 0; i < nt; ++i) {
+        const TriRec& t = bvh.tris[i];
+        for (int a = 0; a < 3; ++a) {
+            float v0 = t.m[a], v1 = t.m[a] + t.e[a], v2 = t.m[a] + t.f[a];
+            lo[3 * i + a] = std::min({v0, v1, v2}) - grow;
+            hi[3 * i + a] = std::max({v0, v1, v2}) + grow;
+        }
+        all.grow(&lo[3 * i]);
+        all.grow(&hi[3 * i]);
+    }
+    int64_t ncell = 1;
+    for (int a = 0; a < 3; ++a) {
+        g.origin[a] = all.lo[a];
+        g.n[a] = std::max<int32_t>(1, (int32_t)std::ceil(((double)all.hi[a] - all.lo[a]) / cell) + 1);
+        g.inv_cell[a] = 1.0f / cell;
+        ncell *= g.n[a];
+    }
+    if (ncell > (int64_t)1 << 31) return NearGrid();
+    g.r0 = r0;
+    auto crange = [&](int64_t i, int a, int32_t& c0, int32_t& c1) {
+        // cells are found by the same float arithmetic the kernel uses, one
+        // cell of slack on each side
+        float f0 = (lo[3 * i + a] - g.origin[a]) * g.inv_cell[a];
+        float f1 = (hi[3 * i + a] - g.origin[a]) * g.inv_cell[a];
+        c0 = std::max(0, (int32_t)std::floor(f0) - 1);
+        c1 = std::min(g.n[a] - 1, (int32_t)std::floor(f1) + 1);
+    };
+    std::vector<uint32_t> count((size_t)ncell + 1, 0);
+    for (int64_t i = 0; i < nt; ++i) {
+        int32_t c0[3], c1[3];
+        for (int a = 0; a < 3; ++a) crange(i, a, c0[a], c1[a]);
+        for (int32_t z = c0[2]; z <= c1[2]; ++z)
+            for (int32_t y = c0[1]; y <= c1[1]; ++y)
+                for (int32_t x = c0[0]; x <= c1[0]; ++x) count[((int64_t)z * g.n[1] + y) * g.n[0] + x]++;
+    }
+    g.start.assign((size_t)ncell + 1, 0);
+    uint64_t acc = 0;
+    for (int64_t c = 0; c < ncell; ++c) {
+        g.start[c] = (uint32_t)acc;
+        acc += count[c];
+    }
+    if (acc > 0xffffffffull) return NearGrid();
+    g.start[ncell] = (uint32_t)acc;
+    g.list.resize((size_t)acc);
+    std::vector<uint32_t> fill(g.start.begin(), g.start.end() - 1);
+    for (int64_t i = 0; i < nt; ++i) {
+        int32_t c0[3], c1[3];
+        for (int a = 0; a < 3; ++a) crange(i, a, c0[a], c1[a]);
+        for (int32_t z = c0[2]; z <= c1[2]; ++z)
+            for (int32_t y = c0[1]; y <= c1[1]; ++y)
+                for (int32_t x = c0[0]; x <= c1[0]; ++x)
+                    g.list[fill[((int64_t)z * g.n[1] + y) * g.n[0] + x]++] = (int32_t)i;
+    }
+    return g;
+}
+
 }  // namespace lp

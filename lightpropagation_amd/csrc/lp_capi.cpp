@@ -66,6 +66,9 @@ struct lp_context {
     lp_layout L{};
     Bvh bvh;
     SafetyGrid sg;
+    NearGrid ng;
+    float near_r0 = 0.1f, near_cell = 0.0f;  // cell 0 = 2 r0
+    DevBuf d_near_start, d_near_list;
     bool mesh_dirty = true, src_dirty = true, tallies_dirty = true;
 
     // device
@@ -218,6 +221,12 @@ int prepare(lp_context* c)
         for (float x : c->tri) c->bvh_extent = std::max(c->bvh_extent, std::fabs(x));
         int cells = c->safety_cells < 0 ? 0 : c->safety_cells;  // off unless asked: the free radius covers it
         c->sg = cells > 0 ? build_safety_grid(c->bvh, c->tri.data(), c->ntri, cells) : SafetyGrid();
+        c->ng = c->near_r0 > 0 ? build_near_grid(c->bvh, c->near_r0, c->near_cell > 0 ? c->near_cell : 2 * c->near_r0)
+                               : NearGrid();
+        if (!c->ng.list.empty()) {
+            if ((rc = upload(c, c->d_near_start, c->ng.start.data(), c->ng.start.size() * sizeof(uint32_t)))) return rc;
+            if ((rc = upload(c, c->d_near_list, c->ng.list.data(), c->ng.list.size() * sizeof(int32_t)))) return rc;
+        }
         if ((rc = upload(c, c->d_tris, c->bvh.tris.data(), c->bvh.tris.size() * sizeof(TriRec)))) return rc;
         if ((rc = upload(c, c->d_nodes, c->bvh.nodes.data(), c->bvh.nodes.size() * sizeof(BvhNode)))) return rc;
         if ((rc = upload(c, c->d_nrm, c->nrm.data(), c->nrm.size() * sizeof(float)))) return rc;
@@ -256,6 +265,16 @@ void fill_args(lp_context* c, KArgs& a)
     a.nrm = (const float*)c->d_nrm.p;
     a.med = (const int32_t*)c->d_med.p;
     a.ntri = (int32_t)c->ntri;
+    if (!c->ng.list.empty()) {
+        a.near_start = (const uint32_t*)c->d_near_start.p;
+        a.near_list = (const int32_t*)c->d_near_list.p;
+        for (int k = 0; k < 3; ++k) {
+            a.ng_origin[k] = c->ng.origin[k];
+            a.ng_inv[k] = c->ng.inv_cell[k];
+            a.ng_n[k] = c->ng.n[k];
+        }
+        a.near_r0 = c->ng.r0;
+    }
     a.dist_cap = c->dist_cap;
     a.dist_cap_steps = c->dist_cap_steps;
     a.free_iters = c->free_iters;
@@ -382,8 +401,8 @@ void lp_destroy(lp_context* c)
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety,
-                      &c->d_vstart, &c->d_vocc, &c->d_counter, &c->d_exit};
+    DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_start, &c->d_near_list};
     for (DevBuf* b : bufs) dev_free(*b);
     for (auto& b : c->d_tally) dev_free(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -550,6 +569,16 @@ int lp_set_option(lp_context* c, int32_t key, double v)
         if (!(v == 1 || v == 4 || v == 5 || v == 6)) return fail(c, LP_EINVAL, "waves per SIMD must be 1, 4, 5 or 6");
         c->waves = (int32_t)v;
         return LP_OK;
+    case LP_OPT_NEAR_R0:
+        if (!(v >= 0 && v < 1e6)) return fail(c, LP_EINVAL, "near radius must be >= 0");
+        if ((float)v != c->near_r0) c->mesh_dirty = true;
+        c->near_r0 = (float)v;
+        return LP_OK;
+    case LP_OPT_NEAR_CELL:
+        if (!(v >= 0 && v < 1e6)) return fail(c, LP_EINVAL, "near cell must be >= 0");
+        if ((float)v != c->near_cell) c->mesh_dirty = true;
+        c->near_cell = (float)v;
+        return LP_OK;
     case LP_OPT_BLOCKS_PER_CU:
         if (v < 1 || v > 32) return fail(c, LP_EINVAL, "blocks per CU out of range");
         c->blocks_per_cu = (int32_t)v;
@@ -660,8 +689,8 @@ int lp_get_stats(lp_context* c, lp_stats* out)
 {
     if (!c || !out) return LP_EINVAL;
     int64_t bytes = 0;
-    DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety,
-                      &c->d_vstart, &c->d_vocc, &c->d_counter, &c->d_exit};
+    DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_start, &c->d_near_list};
     for (DevBuf* b : bufs) bytes += (int64_t)b->bytes;
     for (auto& b : c->d_tally) bytes += (int64_t)b.bytes;
     c->stats.device_bytes = bytes;
@@ -691,6 +720,31 @@ int lp_debug_math(lp_context* c, int32_t op, const float* xy, float* out, int64_
         rc = fail(c, LP_EDEVICE, "lp_debug_math failed");
     dev_free(din);
     dev_free(dout);
+    return rc;
+}
+
+int lp_debug_copy(lp_context* c, int64_t bytes, double* ms)
+{
+    if (!c || bytes < 16 || bytes % 16) return c ? fail(c, LP_EINVAL, "lp_debug_copy: bytes must be a multiple of 16")
+                                                 : LP_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf s, d;
+    int rc;
+    if ((rc = dev_alloc(c, s, (size_t)bytes)) || (rc = dev_alloc(c, d, (size_t)bytes))) {
+        dev_free(s);
+        dev_free(d);
+        return rc;
+    }
+    rc = LP_OK;
+    float t = 0;
+    if (hipMemsetAsync(s.p, 1, (size_t)bytes, c->stream) != hipSuccess ||
+        hipEventRecord(c->ev0, c->stream) != hipSuccess ||
+        launch_debug_copy(s.p, d.p, bytes, c->stream) != 0 || hipEventRecord(c->ev1, c->stream) != hipSuccess ||
+        hipEventSynchronize(c->ev1) != hipSuccess || hipEventElapsedTime(&t, c->ev0, c->ev1) != hipSuccess)
+        rc = fail(c, LP_EDEVICE, "lp_debug_copy failed");
+    if (ms) *ms = t;
+    dev_free(s);
+    dev_free(d);
     return rc;
 }
 

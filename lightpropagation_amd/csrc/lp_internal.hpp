@@ -56,6 +56,22 @@ struct SafetyGrid {
 };
 SafetyGrid build_safety_grid(const Bvh& bvh, const float* tri, int64_t ntri, int target_cells_per_axis);
 
+// Near-face lists: a uniform grid whose cell c lists (leaf-order indices of)
+// every face whose bounding box, widened by r0 and the rounding pad, meets
+// the cell.  Any face within r0 of a point in c is therefore in c's list, so
+// a nearest-face query of length <= r0 and a distance query capped at r0 are
+// answered exactly by one pass over that list.  A point outside the grid is
+// more than r0 from every face.
+struct NearGrid {
+    float origin[3] = {0, 0, 0};
+    float inv_cell[3] = {0, 0, 0};
+    int32_t n[3] = {0, 0, 0};
+    float r0 = 0.0f;
+    std::vector<uint32_t> start;  // ncell + 1
+    std::vector<int32_t> list;
+};
+NearGrid build_near_grid(const Bvh& bvh, float r0, float cell);
+
 constexpr int kMaxTissues = 64;
 constexpr int kMaxDetectors = 64;
 constexpr int kBlock = 256;
@@ -73,6 +89,11 @@ struct KArgs {
     const float* safety;  // optional safety grid
     float sg_origin[3], sg_inv[3];
     int32_t sg_n[3];
+    const uint32_t* near_start;  // near-face lists (null = off)
+    const int32_t* near_list;
+    float ng_origin[3], ng_inv[3];
+    int32_t ng_n[3];
+    float near_r0;
     float dist_cap;        // per-photon free-radius search: cap (0 = off)
     float dist_cap_steps;  // ... and cap in units of the current step length
     float safe_margin;     // absolute rounding margin of the free radius
@@ -114,11 +135,12 @@ struct KArgs {
 enum MiscSlot {
     kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
     kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscDistQueries,
-    kMiscDistVisits, kMiscN = 16
+    kMiscDistVisits, kMiscListTests, kMiscN = 16
 };
 
 // Launchers (defined in lp_kernel.hip).
 int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen);
 int launch_debug_math(int op, const float* in, float* out, int64_t n, void* stream);
+int launch_debug_copy(const void* src, void* dst, int64_t bytes, void* stream);
 
 }  // namespace lp

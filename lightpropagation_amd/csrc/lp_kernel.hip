@@ -310,6 +310,63 @@ __device__ __forceinline__ float mesh_distance(const KArgs& a, const float4* lno
     return __builtin_sqrtf(best2);
 }
 
+// Cell of the near-face grid holding p, or -1 outside it (then no face lies
+// within near_r0 of p).
+__device__ __forceinline__ int64_t near_cell(const KArgs& a, f3 p)
+{
+    float fx = (p.x - a.ng_origin[0]) * a.ng_inv[0];
+    float fy = (p.y - a.ng_origin[1]) * a.ng_inv[1];
+    float fz = (p.z - a.ng_origin[2]) * a.ng_inv[2];
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f)) return -1;
+    int32_t ix = (int32_t)fminf(fx, 2.0e9f), iy = (int32_t)fminf(fy, 2.0e9f), iz = (int32_t)fminf(fz, 2.0e9f);
+    if (ix >= a.ng_n[0] || iy >= a.ng_n[1] || iz >= a.ng_n[2]) return -1;
+    return ((int64_t)iz * a.ng_n[1] + iy) * a.ng_n[0] + ix;
+}
+
+// One pass over the near-face list of p's cell: the nearest face along
+// p + t v with t <= tmax (tmax <= near_r0; the exact test and tie rule of
+// find_face) and, when want_dist, the squared distance from p to the nearest
+// listed face (+inf when none; faces not listed are farther than near_r0).
+__device__ __forceinline__ void near_query(const KArgs& a, f3 p, f3 v, int32_t ref, float tmax, bool want_dist,
+                                           Hit& h, float& dmin2, uint32_t& tests)
+{
+    h.t = tmax;
+    h.id = -1;
+    h.med = 0;
+    dmin2 = __builtin_inff();
+    int64_t c = near_cell(a, p);
+    if (c < 0) return;
+    uint32_t b = a.near_start[c], e = a.near_start[c + 1];
+    const float4* tr = reinterpret_cast<const float4*>(a.tris);
+    for (uint32_t k = b; k < e; ++k) {
+        int32_t ti = a.near_list[k];
+        float4 A = tr[3 * (int64_t)ti], B = tr[3 * (int64_t)ti + 1], C = tr[3 * (int64_t)ti + 2];
+        tests++;
+        f3 M = mk(A.x, A.y, A.z), E = mk(B.x, B.y, B.z), F = mk(C.x, C.y, C.z);
+        if (want_dist) dmin2 = fminf(dmin2, point_tri_dist2(p, M, E, F));
+        int32_t id = __float_as_int(A.w);
+        if (id == ref) continue;
+        f3 G = sub(p, M);
+        f3 GxE = cross(G, E), vxF = cross(v, F);
+        float w = dot(vxF, E);
+        if (w == 0.0f) continue;
+        float tx = dot(GxE, F);
+        bool wp = w > 0.0f;
+        if ((tx > 0.0f) != wp) continue;
+        float ty = dot(vxF, G), tz = dot(GxE, v);
+        float tiny = __builtin_fabsf(w) * 1e-30f;
+        if (!(ty == 0.0f || (ty > 0.0f) == wp || __builtin_fabsf(ty) < tiny)) continue;
+        if (!(tz == 0.0f || (tz > 0.0f) == wp || __builtin_fabsf(tz) < tiny)) continue;
+        float t = tx / w, kk = ty / w, l = tz / w;
+        if (t > 0.0f && kk >= 0.0f && l >= 0.0f && (kk + l) < 1.0f && t < 10000.0f &&
+            (t < h.t || (t == h.t && (h.id < 0 || id < h.id)))) {
+            h.t = t;
+            h.id = id;
+            h.med = __float_as_int(B.w);
+        }
+    }
+}
+
 __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v)
 {
     if (v) atomicAdd(p, v);
@@ -364,7 +421,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     // events, escaped, capped, skipped searches, distance queries, NaN
     // photons, BVH visits, triangle tests, distance-query visits); flushed
     // to the 64-bit totals when one grows large and at the end
-    enum { kPhot, kSteps, kInt, kBnd, kEsc, kCap, kSkip, kDq, kNan, kVis, kTst, kDvis, kNc };
+    enum { kPhot, kSteps, kInt, kBnd, kEsc, kCap, kSkip, kDq, kNan, kVis, kTst, kDvis, kLst, kNc };
     uint32_t cnt[kNc];
 #pragma unroll
     for (int i = 0; i < kNc; ++i) cnt[i] = 0;
@@ -669,7 +726,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         if (big && a.misc) {  // rare: keep the 32-bit counters from wrapping
             const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
                                    kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
-                                   kMiscDistVisits};
+                                   kMiscDistVisits, kMiscListTests};
 #pragma unroll
             for (int i = 0; i < kNc; ++i) {
                 atomic_add_u64(&a.misc[slot[i]], cnt[i]);
@@ -776,8 +833,10 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 // phase B: refresh the free radius of waiting lanes -- the
                 // nearest-face distance less a rounding margin (a face found by
                 // the exact test at t <= s lies within s + margin of p)
-                bool dq = search && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && __builtin_isfinite(ph.p.x) &&
-                          __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
+                const bool finite_p =
+                    __builtin_isfinite(ph.p.x) && __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
+                const bool use_list = a.near_start && !ballistic && stepsize <= a.near_r0 && finite_p;
+                bool dq = search && !use_list && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && finite_p;
                 if (__any(dq)) {
                     if (dq) {
                         float cap = fminf(a.dist_cap, a.dist_cap_steps * stepsize);
@@ -796,8 +855,22 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 // phase C: nearest face within the pending step
                 if (__any(search)) {
                     if (search) {
-                        find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, cnt[kVis],
-                                  cnt[kTst]);
+                        if (use_list) {
+                            // exact answer from the cell's list, and the free radius with it
+                            float dmin2;
+                            bool want = ph.ref < 0;
+                            near_query(a, ph.p, ph.v, ph.ref, stepsize, want, h, dmin2, cnt[kLst]);
+                            if (want) {
+                                float d = fminf(__builtin_sqrtf(dmin2), a.near_r0);
+                                float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
+                                                  __builtin_fabsf(ph.p.z));
+                                ph.safe = d - (a.safe_margin + 1e-5f * (d + rad));
+                                cnt[kDq]++;
+                            }
+                        } else {
+                            find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, cnt[kVis],
+                                      cnt[kTst]);
+                        }
                         bnd = h.id >= 0 || ballistic || stepsize >= 10000.0f;  // no face ahead at all: escape
                         if (h.id < 0) h.t = 10000.0f;
                         if (ballistic) stepsize = h.t;
@@ -825,7 +898,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     if (a.misc) {
         const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
                                kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
-                               kMiscDistVisits};
+                               kMiscDistVisits, kMiscListTests};
 #pragma unroll
         for (int i = 0; i < kNc; ++i) {
             uint64_t v = cnt[i];
@@ -891,6 +964,23 @@ __global__ void debug_math_kernel(int op, const float* in, float* out, int64_t n
     default: r = x + y; break;
     }
     out[i] = r;
+}
+
+// Plain streaming copy (16 B per lane per iteration): the reference point for
+// converting the profiler's FETCH_SIZE / WRITE_SIZE counters into bytes.
+__global__ void debug_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n)
+{
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+int launch_debug_copy(const void* src, void* dst, int64_t bytes, void* stream)
+{
+    int64_t n = bytes / 16;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(debug_copy_kernel, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const float4*)src,
+                       (float4*)dst, n);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int launch_debug_math(int op, const float* in, float* out, int64_t n, void* stream)

@@ -179,8 +179,8 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
 #define LP_OPT_BLOCKS_PER_CU 5  /* work-groups per CU when GRID_BLOCKS is automatic */
 #define LP_OPT_FREE_ITERS 6     /* search-free steps a lane may take per round      */
 #define LP_OPT_WAVES_PER_SIMD 7 /* register budget of the kernel variant: 1 (compiler's
-                                   choice), 4, 5 or 6 waves per SIMD               */
-#define LP_OPT_NEAR_R0 8        /* near-face lists: radius r0 (0 = off, default 0.1) */
+                                   choice), 4 (default), 5 or 6 waves per SIMD     */
+#define LP_OPT_NEAR_R0 8        /* near-face lists: radius r0 (0 = off, default 0.05) */
 #define LP_OPT_NEAR_CELL 9      /* near-face lists: cell edge (0 = 2 r0)            */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
 

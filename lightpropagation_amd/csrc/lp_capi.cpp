@@ -59,7 +59,7 @@ struct lp_context {
     float dist_cap = 0.2f, dist_cap_steps = 16.0f;
     int32_t blocks_per_cu = 4;
     int32_t free_iters = 16;
-    int32_t waves = 1;
+    int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
 
     // derived
@@ -67,7 +67,7 @@ struct lp_context {
     Bvh bvh;
     SafetyGrid sg;
     NearGrid ng;
-    float near_r0 = 0.1f, near_cell = 0.0f;  // cell 0 = 2 r0
+    float near_r0 = 0.05f, near_cell = 0.0f;  // cell 0 = 2 r0
     DevBuf d_near_start, d_near_list;
     bool mesh_dirty = true, src_dirty = true, tallies_dirty = true;
 

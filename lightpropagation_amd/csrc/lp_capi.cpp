@@ -59,6 +59,7 @@ struct lp_context {
     float dist_cap = 0.2f, dist_cap_steps = 16.0f;
     int32_t blocks_per_cu = 4;
     int32_t free_iters = 16;
+    float search_min = 1.0f;
     int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
 
@@ -278,6 +279,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.dist_cap = c->dist_cap;
     a.dist_cap_steps = c->dist_cap_steps;
     a.free_iters = c->free_iters;
+    a.search_min = c->search_min;
     a.waves = c->waves;
     a.safe_margin = 2e-6f * c->bvh_extent;
     if (!c->sg.dist.empty()) {
@@ -578,6 +580,10 @@ int lp_set_option(lp_context* c, int32_t key, double v)
         if (!(v >= 0 && v < 1e6)) return fail(c, LP_EINVAL, "near cell must be >= 0");
         if ((float)v != c->near_cell) c->mesh_dirty = true;
         c->near_cell = (float)v;
+        return LP_OK;
+    case LP_OPT_SEARCH_MIN:
+        if (!(v >= 0 && v <= 1)) return fail(c, LP_EINVAL, "search fraction must be in [0, 1]");
+        c->search_min = (float)v;
         return LP_OK;
     case LP_OPT_BLOCKS_PER_CU:
         if (v < 1 || v > 32) return fail(c, LP_EINVAL, "blocks per CU out of range");

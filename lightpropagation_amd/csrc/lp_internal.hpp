@@ -98,6 +98,7 @@ struct KArgs {
     float dist_cap_steps;  // ... and cap in units of the current step length
     float safe_margin;     // absolute rounding margin of the free radius
     int32_t free_iters;    // search-free steps a lane may take per round
+    float search_min;      // start the search phase when this fraction of live lanes waits
     int32_t waves;         // occupancy target of the kernel variant (1, 4, 5, 6)
     float nmuasg[4 * kMaxTissues];
     int32_t ntissue, fetal_id;
@@ -135,7 +136,7 @@ struct KArgs {
 enum MiscSlot {
     kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
     kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscDistQueries,
-    kMiscDistVisits, kMiscListTests, kMiscN = 16
+    kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscN = 16
 };
 
 // Launchers (defined in lp_kernel.hip).

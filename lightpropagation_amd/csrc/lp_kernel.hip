@@ -421,7 +421,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     // events, escaped, capped, skipped searches, distance queries, NaN
     // photons, BVH visits, triangle tests, distance-query visits); flushed
     // to the 64-bit totals when one grows large and at the end
-    enum { kPhot, kSteps, kInt, kBnd, kEsc, kCap, kSkip, kDq, kNan, kVis, kTst, kDvis, kLst, kNc };
+    enum { kPhot, kSteps, kInt, kBnd, kEsc, kCap, kSkip, kDq, kNan, kVis, kTst, kDvis, kLst, kIterA, kRounds, kNc };
     uint32_t cnt[kNc];
 #pragma unroll
     for (int i = 0; i < kNc; ++i) cnt[i] = 0;
@@ -726,7 +726,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         if (big && a.misc) {  // rare: keep the 32-bit counters from wrapping
             const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
                                    kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
-                                   kMiscDistVisits, kMiscListTests};
+                                   kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds};
 #pragma unroll
             for (int i = 0; i < kNc; ++i) {
                 atomic_add_u64(&a.misc[slot[i]], cnt[i]);
@@ -828,8 +828,14 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     search = true;
                 }
             }
-            const bool search_round = it + 1 >= a.free_iters || !__any(go);
+            // start the search phase when enough of the live lanes wait for
+            // one (or none can move, or the iteration cap is reached)
+            const int nwait = __popcll(__ballot(search));
+            const int nlive = __popcll(__ballot(alive));
+            const bool search_round = it + 1 >= a.free_iters || !__any(go) || nwait >= a.search_min * nlive;
+            if (lane == 0) cnt[kIterA]++;
             if (search_round) {
+                if (lane == 0) cnt[kRounds]++;
                 // phase B: refresh the free radius of waiting lanes -- the
                 // nearest-face distance less a rounding margin (a face found by
                 // the exact test at t <= s lies within s + margin of p)
@@ -898,7 +904,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     if (a.misc) {
         const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
                                kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
-                               kMiscDistVisits, kMiscListTests};
+                               kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds};
 #pragma unroll
         for (int i = 0; i < kNc; ++i) {
             uint64_t v = cnt[i];

@@ -52,7 +52,7 @@ extern "C" {
                              off-grid voxel weight, BVH node visits, triangle
                              tests, skipped searches, NaN photons, free-radius
                              queries, their node visits, list face tests,
-                             wavefront iterations, search rounds (16 slots)   */
+                             wavefront iterations, search rounds (32 slots)   */
 #define LP_NTALLY 9
 
 /* Tally values are int64 fixed point: 2^32 per unit photon weight, except

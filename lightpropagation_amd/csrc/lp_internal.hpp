@@ -133,11 +133,13 @@ struct KArgs {
     unsigned long long* counter;
 };
 
-enum MiscSlot {
+enum MiscSlot {  // slots of the misc tally (kMiscN int64 words)
     kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
     kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscDistQueries,
-    kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscN = 16
+    kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscUsed, kMiscN = 32
 };
+
+static_assert(kMiscUsed <= kMiscN, "misc tally too small for its counters");
 
 // Launchers (defined in lp_kernel.hip).
 int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen);

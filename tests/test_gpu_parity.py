@@ -169,3 +169,12 @@ def test_bench_scene_photons_that_turned_nan():
     assert np.all(np.isfinite(ex[:, :6]))
     assert ctx.stats().kernel_ms < 20000
     ctx.close()
+
+
+def test_misc_counters_fit_their_tally(ctx):
+    sc = _scene("c0slab")
+    _configure(ctx, sc, 1, lp.RNG_PHILOX)
+    assert len(lp.MISC) <= ctx.layout().n[lp.TALLY["misc"]]
+    ctx.run(0, 1000)
+    m = ctx.tallies()["misc"]
+    assert m["photons"] == 1000 and m["wave_iters"] > 0 and m["search_rounds"] > 0

@@ -59,7 +59,7 @@ def build(verbose=False, jobs=None):
         if src.endswith(".hip"):
             cmd = [HIPCC] + COMMON + DEVICE + ["-c", "-o", obj, path]
         else:
-            cmd = [HIPCC] + COMMON + ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", "-o", obj, path]
+            cmd = [HIPCC] + COMMON + ["-x", "c++", "-pthread", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", "-o", obj, path]
         if verbose:
             print(" ".join(shlex.quote(c) for c in cmd), flush=True)
         procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True), src))
@@ -75,7 +75,7 @@ def build(verbose=False, jobs=None):
     if failed:
         raise RuntimeError("build failed: " + ", ".join(failed))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lamdhip64"], verbose)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs + ["-lamdhip64"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <deque>
 #include <numeric>
+#include <thread>
 
 #include "lp_internal.hpp"
 
@@ -316,67 +317,212 @@ SafetyGrid build_safety_grid(const Bvh& bvh, const float* tri, int64_t ntri, int
     return g;
 }
 
+namespace {
+
+// Squared distance from p to the triangle (a, a + ab, a + ac), in double
+// (closest-point regions of Ericson, Real-Time Collision Detection 5.1.5).
+double point_tri_dist2_d(const double p[3], const double a[3], const double ab[3], const double ac[3])
+{
+    auto dt = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    double ap[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+    double d1 = dt(ab, ap), d2 = dt(ac, ap);
+    double q[3];
+    auto len2 = [&]() { return dt(q, q); };
+    if (d1 <= 0 && d2 <= 0) return dt(ap, ap);
+    double bp[3] = {ap[0] - ab[0], ap[1] - ab[1], ap[2] - ab[2]};
+    double d3 = dt(ab, bp), d4 = dt(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return dt(bp, bp);
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+        double v = d1 / (d1 - d3);
+        for (int i = 0; i < 3; ++i) q[i] = ap[i] - v * ab[i];
+        return len2();
+    }
+    double cp[3] = {ap[0] - ac[0], ap[1] - ac[1], ap[2] - ac[2]};
+    double d5 = dt(ab, cp), d6 = dt(ac, cp);
+    if (d6 >= 0 && d5 <= d6) return dt(cp, cp);
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+        double w = d2 / (d2 - d6);
+        for (int i = 0; i < 3; ++i) q[i] = ap[i] - w * ac[i];
+        return len2();
+    }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int i = 0; i < 3; ++i) q[i] = bp[i] - w * (ac[i] - ab[i]);
+        return len2();
+    }
+    double den = va + vb + vc;
+    if (!(den != 0)) return 0.0;  // degenerate face: treat as touching
+    double v = vb / den, w = vc / den;
+    for (int i = 0; i < 3; ++i) q[i] = ap[i] - ab[i] * v - ac[i] * w;
+    return len2();
+}
+
+}  // namespace
+
 NearGrid build_near_grid(const Bvh& bvh, float r0, float cell)
 {
     NearGrid g;
     const int64_t nt = (int64_t)bvh.tris.size();
     if (nt == 0 || !(r0 > 0) || !(cell > 0)) return g;
-    // boxes of the stored triangles (M, M + E, M + F), widened by r0 + pad
-    const float grow = r0 + bvh.pad;
-    std::vector<float> lo(3 * nt), hi(3 * nt);
+    // A point that the kernel places in cell c (by the float arithmetic of
+    // near_cell) lies within the cell's box grown by the rounding pad, so it
+    // is within half_diag + 2 pad of the cell centre.  Cell c lists face f
+    // when dist(centre, f) <= r0 + half_diag + 2 pad: every face within r0 of
+    // such a point is listed.  Cell c is "touched" by f when
+    // dist(centre, f) <= half_diag + 2 pad: a face only passes through
+    // touched cells.
+    const double pad = 2.0 * (double)bvh.pad;
+    const double half_diag = 0.5 * std::sqrt(3.0) * (double)cell;
+    const double r_list = (double)r0 + half_diag + pad;
+    const double r_touch = half_diag + pad;
     Box all;
     for (int64_t i = 0; i < nt; ++i) {
         const TriRec& t = bvh.tris[i];
-        for (int a = 0; a < 3; ++a) {
-            float v0 = t.m[a], v1 = t.m[a] + t.e[a], v2 = t.m[a] + t.f[a];
-            lo[3 * i + a] = std::min({v0, v1, v2}) - grow;
-            hi[3 * i + a] = std::max({v0, v1, v2}) + grow;
+        for (int k = 0; k < 3; ++k) {
+            float v[3];
+            for (int a = 0; a < 3; ++a) v[a] = t.m[a] + (k == 1 ? t.e[a] : k == 2 ? t.f[a] : 0.0f);
+            all.grow(v);
         }
-        all.grow(&lo[3 * i]);
-        all.grow(&hi[3 * i]);
     }
     int64_t ncell = 1;
+    const float grow = (float)(r_list + cell);
     for (int a = 0; a < 3; ++a) {
-        g.origin[a] = all.lo[a];
-        g.n[a] = std::max<int32_t>(1, (int32_t)std::ceil(((double)all.hi[a] - all.lo[a]) / cell) + 1);
+        g.origin[a] = all.lo[a] - grow;
+        g.n[a] = std::max<int32_t>(1, (int32_t)std::ceil(((double)all.hi[a] - all.lo[a] + 2.0 * grow) / cell) + 1);
         g.inv_cell[a] = 1.0f / cell;
         ncell *= g.n[a];
     }
-    if (ncell > (int64_t)1 << 31) return NearGrid();
+    if (ncell > ((int64_t)1 << 31) - 1 || nt > ((int64_t)1 << 31) - 1) return NearGrid();
     g.r0 = r0;
-    auto crange = [&](int64_t i, int a, int32_t& c0, int32_t& c1) {
-        // cells are found by the same float arithmetic the kernel uses, one
-        // cell of slack on each side
-        float f0 = (lo[3 * i + a] - g.origin[a]) * g.inv_cell[a];
-        float f1 = (hi[3 * i + a] - g.origin[a]) * g.inv_cell[a];
-        c0 = std::max(0, (int32_t)std::floor(f0) - 1);
-        c1 = std::min(g.n[a] - 1, (int32_t)std::floor(f1) + 1);
+    // cell centres follow the float indexing of near_cell: centre of cell i
+    // along axis a = origin + (i + 0.5) * cell
+    auto centre = [&](int a, int32_t i) { return (double)g.origin[a] + ((double)i + 0.5) * (double)cell; };
+
+    // (cell << 32) | face | touched-bit, generated per face range in threads
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<uint64_t>> pairs(nth);
+    auto work = [&](unsigned w) {
+        int64_t f0 = nt * w / nth, f1 = nt * (w + 1) / nth;
+        std::vector<uint64_t>& out = pairs[w];
+        for (int64_t i = f0; i < f1; ++i) {
+            const TriRec& t = bvh.tris[i];
+            double A[3], AB[3], AC[3], N[3];
+            for (int a = 0; a < 3; ++a) {
+                A[a] = t.m[a];
+                AB[a] = t.e[a];
+                AC[a] = t.f[a];
+            }
+            N[0] = AB[1] * AC[2] - AB[2] * AC[1];
+            N[1] = AB[2] * AC[0] - AB[0] * AC[2];
+            N[2] = AB[0] * AC[1] - AB[1] * AC[0];
+            double nl = std::sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+            int32_t c0[3], c1[3];
+            for (int a = 0; a < 3; ++a) {
+                double lo = std::min({A[a], A[a] + AB[a], A[a] + AC[a]}) - r_list;
+                double hi = std::max({A[a], A[a] + AB[a], A[a] + AC[a]}) + r_list;
+                c0[a] = std::max<int32_t>(0, (int32_t)std::floor((lo - g.origin[a]) / cell) - 1);
+                c1[a] = std::min<int32_t>(g.n[a] - 1, (int32_t)std::floor((hi - g.origin[a]) / cell) + 1);
+            }
+            for (int32_t z = c0[2]; z <= c1[2]; ++z)
+                for (int32_t y = c0[1]; y <= c1[1]; ++y)
+                    for (int32_t x = c0[0]; x <= c1[0]; ++x) {
+                        double P[3] = {centre(0, x), centre(1, y), centre(2, z)};
+                        if (nl > 0) {  // plane pre-test
+                            double sd = ((P[0] - A[0]) * N[0] + (P[1] - A[1]) * N[1] + (P[2] - A[2]) * N[2]) / nl;
+                            if (std::fabs(sd) > r_list) continue;
+                        }
+                        double d2 = point_tri_dist2_d(P, A, AB, AC);
+                        if (d2 > r_list * r_list) continue;
+                        uint64_t cidx = ((uint64_t)z * g.n[1] + y) * g.n[0] + x;
+                        out.push_back(cidx << 32 | (uint64_t)i | (d2 <= r_touch * r_touch ? 0x80000000ull : 0));
+                    }
+        }
     };
-    std::vector<uint32_t> count((size_t)ncell + 1, 0);
-    for (int64_t i = 0; i < nt; ++i) {
-        int32_t c0[3], c1[3];
-        for (int a = 0; a < 3; ++a) crange(i, a, c0[a], c1[a]);
-        for (int32_t z = c0[2]; z <= c1[2]; ++z)
-            for (int32_t y = c0[1]; y <= c1[1]; ++y)
-                for (int32_t x = c0[0]; x <= c1[0]; ++x) count[((int64_t)z * g.n[1] + y) * g.n[0] + x]++;
+    {
+        std::vector<std::thread> ts;
+        for (unsigned w = 1; w < nth; ++w) ts.emplace_back(work, w);
+        work(0);
+        for (auto& t : ts) t.join();
     }
-    g.start.assign((size_t)ncell + 1, 0);
+    // counting sort by cell; faces stay in increasing order within a cell
+    std::vector<uint32_t> count((size_t)ncell + 1, 0);
+    std::vector<uint8_t> touched((size_t)ncell, 0);
+    uint64_t total = 0;
+    for (auto& v : pairs)
+        for (uint64_t q : v) {
+            count[q >> 32]++;
+            if (q & 0x80000000ull) touched[q >> 32] = 1;
+            ++total;
+        }
+    if (total > 0xffffffffull) return NearGrid();
+    g.cells.assign(2 * ((size_t)ncell + 1), 0);
     uint64_t acc = 0;
     for (int64_t c = 0; c < ncell; ++c) {
-        g.start[c] = (uint32_t)acc;
+        g.cells[2 * c] = g.cells[2 * c + 1] = (uint32_t)acc;
         acc += count[c];
     }
-    if (acc > 0xffffffffull) return NearGrid();
-    g.start[ncell] = (uint32_t)acc;
+    g.cells[2 * ncell] = (uint32_t)acc;
+    std::vector<uint32_t>().swap(count);
     g.list.resize((size_t)acc);
-    std::vector<uint32_t> fill(g.start.begin(), g.start.end() - 1);
-    for (int64_t i = 0; i < nt; ++i) {
-        int32_t c0[3], c1[3];
-        for (int a = 0; a < 3; ++a) crange(i, a, c0[a], c1[a]);
-        for (int32_t z = c0[2]; z <= c1[2]; ++z)
-            for (int32_t y = c0[1]; y <= c1[1]; ++y)
-                for (int32_t x = c0[0]; x <= c1[0]; ++x)
-                    g.list[fill[((int64_t)z * g.n[1] + y) * g.n[0] + x]++] = (int32_t)i;
+    for (auto& v : pairs) {
+        for (uint64_t q : v) g.list[g.cells[2 * (q >> 32) + 1]++] = (int32_t)(q & 0x7fffffffull);  // odd slot = fill cursor
+        std::vector<uint64_t>().swap(v);
+    }
+    // clearance: Chebyshev cell distance d from the touched cells (exact
+    // two-pass chamfer transform over the 26-neighbourhood; each pass takes
+    // the 9 cells of the previous plane and 3 of the previous row as whole
+    // rows, then runs along the row); a point of a cell at distance d >= 2
+    // is at least (d - 1) * cell from every face
+    const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    std::vector<uint16_t> dist((size_t)ncell);
+    for (int64_t c = 0; c < ncell; ++c) dist[c] = touched[c] ? 0 : 0xfffe;
+    std::vector<uint8_t>().swap(touched);
+    auto row = [&](int64_t y, int64_t z) { return dist.data() + (z * ny + y) * nx; };
+    std::vector<uint16_t> tmp((size_t)nx);
+    for (int pass = 0; pass < 2; ++pass) {
+        const int64_t dir = pass == 0 ? 1 : -1;
+        for (int64_t zi = 0; zi < nz; ++zi) {
+            const int64_t z = pass == 0 ? zi : nz - 1 - zi;
+            for (int64_t yi = 0; yi < ny; ++yi) {
+                const int64_t y = pass == 0 ? yi : ny - 1 - yi;
+                uint16_t* r = row(y, z);
+                // previous rows: (y - dir, z) and the 3 rows of plane z - dir
+                const uint16_t* prev[4];
+                int np = 0;
+                if (y - dir >= 0 && y - dir < ny) prev[np++] = row(y - dir, z);
+                if (z - dir >= 0 && z - dir < nz)
+                    for (int64_t dy = -1; dy <= 1; ++dy)
+                        if (y + dy >= 0 && y + dy < ny) prev[np++] = row(y + dy, z - dir);
+                if (np) {
+                    for (int64_t x = 0; x < nx; ++x) {
+                        uint16_t m = 0xfffe;
+                        for (int k = 0; k < np; ++k) m = std::min(m, prev[k][x]);
+                        tmp[x] = m;
+                    }
+                    // 3-wide minimum along x, + 1
+                    for (int64_t x = 0; x < nx; ++x) {
+                        uint16_t m = tmp[x];
+                        if (x > 0) m = std::min(m, tmp[x - 1]);
+                        if (x + 1 < nx) m = std::min(m, tmp[x + 1]);
+                        r[x] = std::min<uint16_t>(r[x], (uint16_t)std::min(0xfffe, m + 1));
+                    }
+                }
+                // along the row itself
+                if (dir > 0)
+                    for (int64_t x = 1; x < nx; ++x) r[x] = std::min<uint16_t>(r[x], (uint16_t)std::min(0xfffe, r[x - 1] + 1));
+                else
+                    for (int64_t x = nx - 2; x >= 0; --x)
+                        r[x] = std::min<uint16_t>(r[x], (uint16_t)std::min(0xfffe, r[x + 1] + 1));
+            }
+        }
+    }
+    for (int64_t c = 0; c <= ncell; ++c) {
+        double cl = (c < ncell && dist[c] >= 2 && dist[c] < 0xfffe) ? (dist[c] - 1) * (double)cell * 0.999 - pad : 0.0;
+        float f = (float)std::max(0.0, cl);
+        std::memcpy(&g.cells[2 * c + 1], &f, sizeof f);
     }
     return g;
 }

@@ -69,7 +69,7 @@ struct lp_context {
     SafetyGrid sg;
     NearGrid ng;
     float near_r0 = 0.05f, near_cell = 0.0f;  // cell 0 = 2 r0
-    DevBuf d_near_start, d_near_list;
+    DevBuf d_near_cells, d_near_list;
     bool mesh_dirty = true, src_dirty = true, tallies_dirty = true;
 
     // device
@@ -225,7 +225,7 @@ int prepare(lp_context* c)
         c->ng = c->near_r0 > 0 ? build_near_grid(c->bvh, c->near_r0, c->near_cell > 0 ? c->near_cell : 2 * c->near_r0)
                                : NearGrid();
         if (!c->ng.list.empty()) {
-            if ((rc = upload(c, c->d_near_start, c->ng.start.data(), c->ng.start.size() * sizeof(uint32_t)))) return rc;
+            if ((rc = upload(c, c->d_near_cells, c->ng.cells.data(), c->ng.cells.size() * sizeof(uint32_t)))) return rc;
             if ((rc = upload(c, c->d_near_list, c->ng.list.data(), c->ng.list.size() * sizeof(int32_t)))) return rc;
         }
         if ((rc = upload(c, c->d_tris, c->bvh.tris.data(), c->bvh.tris.size() * sizeof(TriRec)))) return rc;
@@ -267,7 +267,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.med = (const int32_t*)c->d_med.p;
     a.ntri = (int32_t)c->ntri;
     if (!c->ng.list.empty()) {
-        a.near_start = (const uint32_t*)c->d_near_start.p;
+        a.near_cells = (const CellRec*)c->d_near_cells.p;
         a.near_list = (const int32_t*)c->d_near_list.p;
         for (int k = 0; k < 3; ++k) {
             a.ng_origin[k] = c->ng.origin[k];
@@ -334,6 +334,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.misc = (unsigned long long*)c->d_tally[LP_TALLY_MISC].p;
     std::memcpy(a.vox_origin, c->vox_origin, sizeof a.vox_origin);
     std::memcpy(a.vox_size, c->vox_size, sizeof a.vox_size);
+    for (int k = 0; k < 3; ++k) a.vox_inv[k] = 1.0f / c->vox_size[k];
     std::memcpy(a.vox_n, c->vox_n, sizeof a.vox_n);
     a.rad_dr = c->rad_dr;
     a.rad_n = c->rad_n;
@@ -404,7 +405,7 @@ void lp_destroy(lp_context* c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
-                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_start, &c->d_near_list};
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_list};
     for (DevBuf* b : bufs) dev_free(*b);
     for (auto& b : c->d_tally) dev_free(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -696,7 +697,7 @@ int lp_get_stats(lp_context* c, lp_stats* out)
     if (!c || !out) return LP_EINVAL;
     int64_t bytes = 0;
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
-                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_start, &c->d_near_list};
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_list};
     for (DevBuf* b : bufs) bytes += (int64_t)b->bytes;
     for (auto& b : c->d_tally) bytes += (int64_t)b.bytes;
     c->stats.device_bytes = bytes;

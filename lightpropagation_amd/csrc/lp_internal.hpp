@@ -1,5 +1,6 @@
 // lp_internal.hpp -- data layout shared by the host runtime and the kernels.
 #pragma once
+#include <cstring>
 #include <stdint.h>
 
 #include <vector>
@@ -57,18 +58,35 @@ struct SafetyGrid {
 SafetyGrid build_safety_grid(const Bvh& bvh, const float* tri, int64_t ntri, int target_cells_per_axis);
 
 // Near-face lists: a uniform grid whose cell c lists (leaf-order indices of)
-// every face whose bounding box, widened by r0 and the rounding pad, meets
-// the cell.  Any face within r0 of a point in c is therefore in c's list, so
-// a nearest-face query of length <= r0 and a distance query capped at r0 are
-// answered exactly by one pass over that list.  A point outside the grid is
-// more than r0 from every face.
+// every face within r0 of some point that the kernel's float cell lookup can
+// place in c (exact point-triangle distances from the cell centre, grown by
+// the half diagonal and the rounding pad).  A nearest-face query of length
+// <= r0 and a distance query capped at r0 are therefore answered exactly by
+// one pass over that list.  The grid extends r0 beyond every face, so a point
+// outside it is more than r0 from every face.
+struct alignas(8) CellRec {
+    uint32_t first;       // first list entry of the cell
+    uint32_t clear_bits;  // float bits of the cell's clearance
+};
 struct NearGrid {
     float origin[3] = {0, 0, 0};
     float inv_cell[3] = {0, 0, 0};
     int32_t n[3] = {0, 0, 0};
     float r0 = 0.0f;
-    std::vector<uint32_t> start;  // ncell + 1
+    // per cell c (and one sentinel): cells[2c] = first list entry of c,
+    // cells[2c + 1] = bits of the float clearance of c: no face lies within
+    // that distance of any point of c (from a Chebyshev distance transform
+    // over the cells the faces pass through), so faces missing from c's list
+    // are farther than max(r0, clearance)
+    std::vector<uint32_t> cells;
     std::vector<int32_t> list;
+    uint32_t start(int64_t c) const { return cells[2 * c]; }
+    float clear(int64_t c) const
+    {
+        float f;
+        std::memcpy(&f, &cells[2 * c + 1], sizeof f);
+        return f;
+    }
 };
 NearGrid build_near_grid(const Bvh& bvh, float r0, float cell);
 
@@ -89,7 +107,7 @@ struct KArgs {
     const float* safety;  // optional safety grid
     float sg_origin[3], sg_inv[3];
     int32_t sg_n[3];
-    const uint32_t* near_start;  // near-face lists (null = off)
+    const CellRec* near_cells;  // near-face lists (null = off)
     const int32_t* near_list;
     float ng_origin[3], ng_inv[3];
     int32_t ng_n[3];
@@ -117,7 +135,7 @@ struct KArgs {
     int32_t rng_mode;
     // tallies (null = off)
     unsigned long long *sft, *rtd, *arz, *phirz, *region, *vox, *rad, *det, *misc;
-    float vox_origin[3], vox_size[3];
+    float vox_origin[3], vox_size[3], vox_inv[3];  // vox_inv = 1 / vox_size in float
     int32_t vox_n[3];
     float rad_dr;
     int32_t rad_n;

@@ -325,25 +325,38 @@ __device__ __forceinline__ int64_t near_cell(const KArgs& a, f3 p)
 
 // One pass over the near-face list of p's cell: the nearest face along
 // p + t v with t <= tmax (tmax <= near_r0; the exact test and tie rule of
-// find_face) and, when want_dist, the squared distance from p to the nearest
-// listed face (+inf when none; faces not listed are farther than near_r0).
+// find_face) and, when want_dist, the squared distances from p to the
+// nearest (d1sq, list entry i1) and second-nearest (d2sq) listed faces (+inf
+// when missing); unlisted = a lower bound on the distance to every face not
+// in the list.
 __device__ __forceinline__ void near_query(const KArgs& a, f3 p, f3 v, int32_t ref, float tmax, bool want_dist,
-                                           Hit& h, float& dmin2, uint32_t& tests)
+                                           Hit& h, float& d1sq, float& d2sq, int32_t& i1, float& unlisted,
+                                           uint32_t& tests)
 {
     h.t = tmax;
     h.id = -1;
     h.med = 0;
-    dmin2 = __builtin_inff();
+    d1sq = d2sq = __builtin_inff();
+    i1 = -1;
+    unlisted = a.near_r0;  // the grid reaches r0 beyond every face
     int64_t c = near_cell(a, p);
     if (c < 0) return;
-    uint32_t b = a.near_start[c], e = a.near_start[c + 1];
+    const CellRec cr = a.near_cells[c];
+    const uint32_t b = cr.first, e = a.near_cells[c + 1].first;
+    unlisted = fmaxf(a.near_r0, __uint_as_float(cr.clear_bits));
     const float4* tr = reinterpret_cast<const float4*>(a.tris);
     for (uint32_t k = b; k < e; ++k) {
         int32_t ti = a.near_list[k];
         float4 A = tr[3 * (int64_t)ti], B = tr[3 * (int64_t)ti + 1], C = tr[3 * (int64_t)ti + 2];
         tests++;
         f3 M = mk(A.x, A.y, A.z), E = mk(B.x, B.y, B.z), F = mk(C.x, C.y, C.z);
-        if (want_dist) dmin2 = fminf(dmin2, point_tri_dist2(p, M, E, F));
+        if (want_dist) {
+            float dd = point_tri_dist2(p, M, E, F);
+            bool nearer = dd < d1sq;
+            d2sq = nearer ? d1sq : fminf(d2sq, dd);
+            i1 = nearer ? ti : i1;
+            d1sq = nearer ? dd : d1sq;
+        }
         int32_t id = __float_as_int(A.w);
         if (id == ref) continue;
         f3 G = sub(p, M);
@@ -378,7 +391,12 @@ __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned l
 struct Photon {
     f3 p, v, nv, sft;
     float w, unfin, absorbed;
-    float safe;  // no face lies within this distance of p
+    // free region: the ball of radius safe around anc, cut by the
+    // half-space dot(pu, x) > pc (pu = 0, pc = -1: no cut); no face meets
+    // the region, p lies inside it, and it is convex, so a step whose end
+    // point stays inside needs no face search
+    f3 anc, pu;
+    float safe, pc;
     int32_t ref, curr;
     int32_t fetal, roulette;
     int64_t steps;
@@ -440,7 +458,6 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         const float4 cm = lmat[ph.curr];
         ph.p = add(ph.p, mul(s, ph.v));
         ph.ref = -1;
-        ph.safe = ph.safe - s;
         if (cm.y > 0.0f && cm.z > 0.0f) {  // Absorb (.cl:841-847)
             ph.absorbed = ph.w * (cm.y / (cm.y + cm.z));
             ph.w -= ph.absorbed;
@@ -503,9 +520,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             }
         }
         if (a.vox) {
-            float fx = (c.x - a.vox_origin[0]) / a.vox_size[0];
-            float fy = (c.y - a.vox_origin[1]) / a.vox_size[1];
-            float fz = (c.z - a.vox_origin[2]) / a.vox_size[2];
+            float fx = (c.x - a.vox_origin[0]) * a.vox_inv[0];
+            float fy = (c.y - a.vox_origin[1]) * a.vox_inv[1];
+            float fz = (c.z - a.vox_origin[2]) * a.vox_inv[2];
             bool ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f;
             int32_t ix = cvt_i32f(fx), iy = cvt_i32f(fy), iz = cvt_i32f(fz);
             ok = ok && ix >= 0 && iy >= 0 && iz >= 0 && ix < a.vox_n[0] && iy < a.vox_n[1] && iz < a.vox_n[2];
@@ -777,6 +794,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                         ph.roulette = 1;
                         ph.steps = 0;
                         ph.safe = 0.0f;
+                        ph.anc = ph.p;
                         ph.k = (int64_t)k;
                         reg_acc = 0;
                         alive = true;
@@ -797,7 +815,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         // together, after up to free_iters search-free iterations (or as soon
         // as no lane can move without a search).
         for (int it = 0;; ++it) {
-            bool go = false;  // this lane takes a step in this iteration
+            bool go = false;     // this lane takes a step in this iteration
+            bool freed = false;  // ... and the step was shown to stay in the free region
             bool bnd = false;
             Hit h;
             h.id = -1;
@@ -821,7 +840,11 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     }
                     have_step = true;
                 }
-                if (!ballistic && (stepsize < ph.safe || (a.safety && stepsize < safety_at(a, ph.p)))) {
+                // free step: the end point stays inside the free region
+                const f3 q = add(ph.p, mul(stepsize, ph.v));
+                const f3 e = sub(q, ph.anc);
+                freed = !ballistic && dot(e, e) < ph.safe * ph.safe && dot(ph.pu, q) > ph.pc;
+                if (freed || (!ballistic && a.safety && stepsize < safety_at(a, ph.p))) {
                     go = true;
                     cnt[kSkip]++;
                 } else {
@@ -841,7 +864,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 // the exact test at t <= s lies within s + margin of p)
                 const bool finite_p =
                     __builtin_isfinite(ph.p.x) && __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
-                const bool use_list = a.near_start && !ballistic && stepsize <= a.near_r0 && finite_p;
+                const bool use_list = a.near_cells && !ballistic && stepsize <= a.near_r0 && finite_p;
                 bool dq = search && !use_list && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && finite_p;
                 if (__any(dq)) {
                     if (dq) {
@@ -849,7 +872,10 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                         float d = mesh_distance(a, lnodes, ph.p, cap, cnt[kDvis]);
                         float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
                                           __builtin_fabsf(ph.p.z));
-                        ph.safe = d - (a.safe_margin + 1e-5f * (d + rad));
+                        ph.safe = fmaxf(0.0f, d - (a.safe_margin + 1e-5f * (d + rad)));  // >= 0: the test squares it
+                        ph.anc = ph.p;
+                        ph.pu = mk(0.0f, 0.0f, 0.0f);
+                        ph.pc = -1.0f;
                         cnt[kDq]++;
                         if (stepsize < ph.safe) {
                             search = false;
@@ -863,14 +889,37 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     if (search) {
                         if (use_list) {
                             // exact answer from the cell's list, and the free radius with it
-                            float dmin2;
+                            float d1sq, d2sq, unlisted;
+                            int32_t i1;
                             bool want = ph.ref < 0;
-                            near_query(a, ph.p, ph.v, ph.ref, stepsize, want, h, dmin2, cnt[kLst]);
+                            near_query(a, ph.p, ph.v, ph.ref, stepsize, want, h, d1sq, d2sq, i1, unlisted, cnt[kLst]);
                             if (want) {
-                                float d = fminf(__builtin_sqrtf(dmin2), a.near_r0);
-                                float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
-                                                  __builtin_fabsf(ph.p.z));
-                                ph.safe = d - (a.safe_margin + 1e-5f * (d + rad));
+                                const float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
+                                                        __builtin_fabsf(ph.p.z));
+                                const float margin = a.safe_margin + 1e-5f * (a.near_r0 + rad);
+                                // cut by the nearest face's plane when p is
+                                // clearly on one side of it: the ball may then
+                                // reach the second-nearest face
+                                float d = fminf(__builtin_sqrtf(d1sq), unlisted);
+                                ph.pu = mk(0.0f, 0.0f, 0.0f);
+                                ph.pc = -1.0f;
+                                if (i1 >= 0 && d1sq < unlisted * unlisted) {
+                                    const float4* tr = reinterpret_cast<const float4*>(a.tris);
+                                    const float4 A = tr[3 * (int64_t)i1], B = tr[3 * (int64_t)i1 + 1],
+                                                 C = tr[3 * (int64_t)i1 + 2];
+                                    const f3 n = cross(mk(B.x, B.y, B.z), mk(C.x, C.y, C.z));
+                                    const f3 u = mul(1.0f / __builtin_sqrtf(dot(n, n)), n);
+                                    const float c0 = dot(u, mk(A.x, A.y, A.z));
+                                    const float sd = dot(u, ph.p) - c0;
+                                    if (__builtin_fabsf(sd) > 4.0f * margin) {  // false for a degenerate face (NaN)
+                                        const float sg = sd > 0.0f ? 1.0f : -1.0f;
+                                        ph.pu = mul(sg, u);
+                                        ph.pc = sg * c0 + margin;
+                                        d = fminf(__builtin_sqrtf(d2sq), unlisted);
+                                    }
+                                }
+                                ph.safe = fmaxf(0.0f, d - (a.safe_margin + 1e-5f * (d + rad)));  // >= 0: the test squares it
+                                ph.anc = ph.p;
                                 cnt[kDq]++;
                             }
                         } else {
@@ -888,10 +937,16 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             // ---- the step itself (the reference loop body, .cl:1375-1575) ----
             if (go) {
                 int kind = 0;
-                if (bnd)
+                if (bnd) {
                     kind = boundary_event(h, stepsize, h.t);
-                else
+                } else {
                     interaction(stepsize);
+                    // keep "p lies in the free region" true: a searched step
+                    // may end outside it, and a later step from outside could
+                    // cross a face even though it ends inside
+                    const f3 e = sub(ph.p, ph.anc);
+                    if (!freed && !(dot(e, e) < ph.safe * ph.safe && dot(ph.pu, ph.p) > ph.pc)) ph.safe = 0.0f;
+                }
                 have_step = false;
                 if (!end_step(kind, bnd ? h.id : -1, stepsize)) {
                     epilogue();

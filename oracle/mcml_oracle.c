@@ -507,9 +507,10 @@ static void record_absorption(const orc_params *P, const orc_layout *L, orc_tall
         }
     }
     if (T->vox && L->n_vox) {
-        float fx = (c.x - P->vox_origin[0]) / P->vox_size[0];
-        float fy = (c.y - P->vox_origin[1]) / P->vox_size[1];
-        float fz = (c.z - P->vox_origin[2]) / P->vox_size[2];
+        /* voxel index = trunc((c - origin) * (1/size)), 1/size in float */
+        float fx = (c.x - P->vox_origin[0]) * (1.0f / P->vox_size[0]);
+        float fy = (c.y - P->vox_origin[1]) * (1.0f / P->vox_size[1]);
+        float fz = (c.z - P->vox_origin[2]) * (1.0f / P->vox_size[2]);
         int ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f;
         int32_t ix = orc_cvt(fx), iy = orc_cvt(fy), iz = orc_cvt(fz);
         ok = ok && ix >= 0 && iy >= 0 && iz >= 0 && ix < P->vox_n[0] && iy < P->vox_n[1] && iz < P->vox_n[2];

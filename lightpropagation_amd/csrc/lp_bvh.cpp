@@ -457,7 +457,7 @@ NearGrid build_near_grid(const Bvh& bvh, float r0, float cell)
             if (q & 0x80000000ull) touched[q >> 32] = 1;
             ++total;
         }
-    if (total > 0xffffffffull) return NearGrid();
+    if (total > 0x7fffffffull) return NearGrid();  // list positions are int32 in the kernel
     g.cells.assign(2 * ((size_t)ncell + 1), 0);
     uint64_t acc = 0;
     for (int64_t c = 0; c < ncell; ++c) {

@@ -69,7 +69,7 @@ struct lp_context {
     SafetyGrid sg;
     NearGrid ng;
     float near_r0 = 0.05f, near_cell = 0.0f;  // cell 0 = 2 r0
-    DevBuf d_near_cells, d_near_list;
+    DevBuf d_near_cells, d_near_tris;
     bool mesh_dirty = true, src_dirty = true, tallies_dirty = true;
 
     // device
@@ -226,7 +226,11 @@ int prepare(lp_context* c)
                                : NearGrid();
         if (!c->ng.list.empty()) {
             if ((rc = upload(c, c->d_near_cells, c->ng.cells.data(), c->ng.cells.size() * sizeof(uint32_t)))) return rc;
-            if ((rc = upload(c, c->d_near_list, c->ng.list.data(), c->ng.list.size() * sizeof(int32_t)))) return rc;
+            // the lists hold copies of the triangle records, so a query reads
+            // one contiguous run of records instead of chasing face indices
+            std::vector<TriRec> recs(c->ng.list.size());
+            for (size_t k = 0; k < recs.size(); ++k) recs[k] = c->bvh.tris[c->ng.list[k]];
+            if ((rc = upload(c, c->d_near_tris, recs.data(), recs.size() * sizeof(TriRec)))) return rc;
         }
         if ((rc = upload(c, c->d_tris, c->bvh.tris.data(), c->bvh.tris.size() * sizeof(TriRec)))) return rc;
         if ((rc = upload(c, c->d_nodes, c->bvh.nodes.data(), c->bvh.nodes.size() * sizeof(BvhNode)))) return rc;
@@ -268,7 +272,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.ntri = (int32_t)c->ntri;
     if (!c->ng.list.empty()) {
         a.near_cells = (const CellRec*)c->d_near_cells.p;
-        a.near_list = (const int32_t*)c->d_near_list.p;
+        a.near_tris = (const TriRec*)c->d_near_tris.p;
         for (int k = 0; k < 3; ++k) {
             a.ng_origin[k] = c->ng.origin[k];
             a.ng_inv[k] = c->ng.inv_cell[k];
@@ -405,7 +409,7 @@ void lp_destroy(lp_context* c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
-                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_list};
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris};
     for (DevBuf* b : bufs) dev_free(*b);
     for (auto& b : c->d_tally) dev_free(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -697,7 +701,7 @@ int lp_get_stats(lp_context* c, lp_stats* out)
     if (!c || !out) return LP_EINVAL;
     int64_t bytes = 0;
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
-                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_list};
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris};
     for (DevBuf* b : bufs) bytes += (int64_t)b->bytes;
     for (auto& b : c->d_tally) bytes += (int64_t)b.bytes;
     c->stats.device_bytes = bytes;

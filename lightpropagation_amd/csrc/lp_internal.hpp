@@ -108,7 +108,7 @@ struct KArgs {
     float sg_origin[3], sg_inv[3];
     int32_t sg_n[3];
     const CellRec* near_cells;  // near-face lists (null = off)
-    const int32_t* near_list;
+    const TriRec* near_tris;  // the triangle records of every cell's list, cell after cell
     float ng_origin[3], ng_inv[3];
     int32_t ng_n[3];
     float near_r0;

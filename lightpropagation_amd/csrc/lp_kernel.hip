@@ -36,29 +36,54 @@ struct Ran3Lds {
 // random numbers
 // ---------------------------------------------------------------------------
 struct PhiloxState {
-    uint32_t id0, id1, block;  // photon index and the next counter block
-    uint32_t b0, b1, b2, b3;   // unused words of the current block, b0 first
-    int left;                  // how many of b0..b3 are unused
-    uint32_t k0, k1;           // the seed
-    __device__ void init(uint64_t seed, uint64_t photon)
+    uint32_t id0, id1, block;                  // photon index and the next counter block
+    uint32_t b0, b1, b2, b3, b4, b5, b6;       // unused words, b0 first
+    int left;                                  // how many of b0..b6 are unused
+    __device__ void init(uint64_t photon)
     {
-        k0 = (uint32_t)seed;
-        k1 = (uint32_t)(seed >> 32);
         id0 = (uint32_t)photon;
         id1 = (uint32_t)(photon >> 32);
         block = 0;
         left = 0;
     }
-    __device__ __forceinline__ float next()
+    __device__ __forceinline__ void gen(uint32_t c[4], uint32_t k0, uint32_t k1)
     {
-        if (left == 0) {
-            uint32_t c[4] = {block, id0, id1, 0u};
-            philox10(c, k0, k1);
+        c[0] = block;
+        c[1] = id0;
+        c[2] = id1;
+        c[3] = 0u;
+        philox10(c, k0, k1);
+        block++;
+    }
+    // Leaves at least 4 unused words: one block is appended when fewer
+    // remain.  Called once per wavefront iteration, so the wavefront runs one
+    // Philox block per iteration instead of one at every draw site at which
+    // some lane has run dry.  Registers only (value selects, no indexing).
+    __device__ __forceinline__ void top_up(uint32_t k0, uint32_t k1)
+    {
+        if (left < 4) {
+            uint32_t c[4];
+            gen(c, k0, k1);
+            const int l = left;
+            b0 = l == 0 ? c[0] : b0;
+            b1 = l == 0 ? c[1] : l == 1 ? c[0] : b1;
+            b2 = l == 0 ? c[2] : l == 1 ? c[1] : l == 2 ? c[0] : b2;
+            b3 = l == 0 ? c[3] : l == 1 ? c[2] : l == 2 ? c[1] : c[0];
+            b4 = l == 1 ? c[3] : l == 2 ? c[2] : l == 3 ? c[1] : b4;
+            b5 = l == 2 ? c[3] : l == 3 ? c[2] : b5;
+            b6 = l == 3 ? c[3] : b6;
+            left = l + 4;
+        }
+    }
+    __device__ __forceinline__ float next(uint32_t k0, uint32_t k1)
+    {
+        if (left == 0) {  // more draws in one iteration than top_up provides (rare)
+            uint32_t c[4];
+            gen(c, k0, k1);
             b0 = c[0];
             b1 = c[1];
             b2 = c[2];
             b3 = c[3];
-            block++;
             left = 4;
         }
         // take b0 and shift: no runtime-indexed array (it would live in scratch)
@@ -66,6 +91,9 @@ struct PhiloxState {
         b0 = b1;
         b1 = b2;
         b2 = b3;
+        b3 = b4;
+        b4 = b5;
+        b5 = b6;
         left--;
         return (float)(u >> 8) * 5.96046448e-08f;
     }
@@ -120,6 +148,7 @@ __device__ __forceinline__ float ran3f_next(int32_t* m, int tid, int& inext, int
 template <int RNG, bool COUNT>
 struct Rng {
     PhiloxState ph;
+    uint32_t k0, k1;  // Philox key = the seed (the same for every lane)
     int inext, inextp;
     int32_t* s;
     int tid;
@@ -128,14 +157,18 @@ struct Rng {
     {
         if (COUNT) ndraw = 0;
         if (RNG == 0)
-            ph.init(seed, photon);
+            ph.init(photon);
         else
             ran3_init_lds(s, tid, seed, photon, inext, inextp);
+    }
+    __device__ __forceinline__ void top_up()
+    {
+        if (RNG == 0) ph.top_up(k0, k1);
     }
     __device__ __forceinline__ float next()
     {
         if (COUNT) ndraw++;
-        if (RNG == 0) return ph.next();
+        if (RNG == 0) return ph.next(k0, k1);
         return ran3f_next(s, tid, inext, inextp);
     }
 };
@@ -344,17 +377,16 @@ __device__ __forceinline__ void near_query(const KArgs& a, f3 p, f3 v, int32_t r
     const CellRec cr = a.near_cells[c];
     const uint32_t b = cr.first, e = a.near_cells[c + 1].first;
     unlisted = fmaxf(a.near_r0, __uint_as_float(cr.clear_bits));
-    const float4* tr = reinterpret_cast<const float4*>(a.tris);
+    const float4* tr = reinterpret_cast<const float4*>(a.near_tris);
     for (uint32_t k = b; k < e; ++k) {
-        int32_t ti = a.near_list[k];
-        float4 A = tr[3 * (int64_t)ti], B = tr[3 * (int64_t)ti + 1], C = tr[3 * (int64_t)ti + 2];
+        float4 A = tr[3 * (int64_t)k], B = tr[3 * (int64_t)k + 1], C = tr[3 * (int64_t)k + 2];
         tests++;
         f3 M = mk(A.x, A.y, A.z), E = mk(B.x, B.y, B.z), F = mk(C.x, C.y, C.z);
         if (want_dist) {
             float dd = point_tri_dist2(p, M, E, F);
             bool nearer = dd < d1sq;
             d2sq = nearer ? d1sq : fminf(d2sq, dd);
-            i1 = nearer ? ti : i1;
+            i1 = nearer ? (int32_t)k : i1;
             d1sq = nearer ? dd : d1sq;
         }
         int32_t id = __float_as_int(A.w);
@@ -432,6 +464,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     Rng<RNG, TRACE> rng;
     rng.s = lran3;
     rng.tid = tid;
+    rng.k0 = (uint32_t)a.seed;
+    rng.k1 = (uint32_t)(a.seed >> 32);
     Photon ph;
     bool alive = false;
     bool exhausted = false;  // wave-uniform
@@ -821,6 +855,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             Hit h;
             h.id = -1;
             h.t = 10000.0f;
+            if (alive) rng.top_up();
             bool act = alive && !search;
             if (act) {
                 if (!have_step) {
@@ -904,7 +939,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                                 ph.pu = mk(0.0f, 0.0f, 0.0f);
                                 ph.pc = -1.0f;
                                 if (i1 >= 0 && d1sq < unlisted * unlisted) {
-                                    const float4* tr = reinterpret_cast<const float4*>(a.tris);
+                                    const float4* tr = reinterpret_cast<const float4*>(a.near_tris);
                                     const float4 A = tr[3 * (int64_t)i1], B = tr[3 * (int64_t)i1 + 1],
                                                  C = tr[3 * (int64_t)i1 + 2];
                                     const f3 n = cross(mk(B.x, B.y, B.z), mk(C.x, C.y, C.z));

@@ -164,7 +164,7 @@ def main():
                              (F_INT, F_BND, search, T)},
         "counters": {k: m[k] for k in ("interact", "boundary", "escaped", "capped", "nan", "node_visits",
                                        "tri_tests", "skipped", "dist_queries", "dist_visits", "list_tests",
-                                       "wave_iters", "search_rounds")},
+                                       "wave_iters", "search_rounds", "bvh_rounds")},
         "options": args.opt,
         "physics": {"R_total": float(R[0]), "F_total": float(R[1]), "T_total": float(R[2]),
                     "detector_R": [float(x) for x in det]},

@@ -52,7 +52,8 @@ extern "C" {
                              off-grid voxel weight, BVH node visits, triangle
                              tests, skipped searches, NaN photons, free-radius
                              queries, their node visits, list face tests,
-                             wavefront iterations, search rounds (32 slots)   */
+                             wavefront iterations, near-list search batches,
+                             BVH search batches (32 slots)                    */
 #define LP_NTALLY 9
 
 /* Tally values are int64 fixed point: 2^32 per unit photon weight, except
@@ -182,8 +183,12 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
                                    choice), 4 (default), 5 or 6 waves per SIMD     */
 #define LP_OPT_NEAR_R0 8        /* near-face lists: radius r0 (0 = off, default 0.05) */
 #define LP_OPT_NEAR_CELL 9      /* near-face lists: cell edge (0 = 2 r0)            */
-#define LP_OPT_SEARCH_MIN 10    /* start a search phase once this fraction of a
+#define LP_OPT_SEARCH_MIN 10    /* run a BVH search batch once this fraction of a
                                    wavefront's live lanes waits for one (0..1)     */
+#define LP_OPT_LIST_MIN 11      /* run a near-list search batch once this fraction
+                                   of a wavefront's live lanes waits for one (0..1) */
+#define LP_OPT_REFILL_MIN 12    /* start new photons once this many lanes of a
+                                   wavefront have finished (1..64)                 */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
 
 #ifdef __cplusplus

@@ -58,8 +58,10 @@ struct lp_context {
     int32_t grid_override = 0, lds_override = -1, safety_cells = -1;
     float dist_cap = 0.2f, dist_cap_steps = 16.0f;
     int32_t blocks_per_cu = 4;
-    int32_t free_iters = 16;
+    int32_t free_iters = 32;
     float search_min = 1.0f;
+    float list_min = 0.25f;
+    int32_t refill_min = 16;
     int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
 
@@ -284,6 +286,8 @@ void fill_args(lp_context* c, KArgs& a)
     a.dist_cap_steps = c->dist_cap_steps;
     a.free_iters = c->free_iters;
     a.search_min = c->search_min;
+    a.list_min = c->list_min;
+    a.refill_min = c->refill_min;
     a.waves = c->waves;
     a.safe_margin = 2e-6f * c->bvh_extent;
     if (!c->sg.dist.empty()) {
@@ -589,6 +593,14 @@ int lp_set_option(lp_context* c, int32_t key, double v)
     case LP_OPT_SEARCH_MIN:
         if (!(v >= 0 && v <= 1)) return fail(c, LP_EINVAL, "search fraction must be in [0, 1]");
         c->search_min = (float)v;
+        return LP_OK;
+    case LP_OPT_LIST_MIN:
+        if (!(v >= 0 && v <= 1)) return fail(c, LP_EINVAL, "list search fraction must be in [0, 1]");
+        c->list_min = (float)v;
+        return LP_OK;
+    case LP_OPT_REFILL_MIN:
+        if (v < 1 || v > 64) return fail(c, LP_EINVAL, "refill lanes must be in [1, 64]");
+        c->refill_min = (int32_t)v;
         return LP_OK;
     case LP_OPT_BLOCKS_PER_CU:
         if (v < 1 || v > 32) return fail(c, LP_EINVAL, "blocks per CU out of range");

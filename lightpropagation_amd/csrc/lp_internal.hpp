@@ -116,7 +116,9 @@ struct KArgs {
     float dist_cap_steps;  // ... and cap in units of the current step length
     float safe_margin;     // absolute rounding margin of the free radius
     int32_t free_iters;    // search-free steps a lane may take per round
-    float search_min;      // start the search phase when this fraction of live lanes waits
+    float search_min;      // run a BVH search batch when this fraction of live lanes waits for one
+    float list_min;        // run a near-list search batch when this fraction of live lanes waits for one
+    int32_t refill_min;    // end the round once this many lanes of a wavefront have finished
     int32_t waves;         // occupancy target of the kernel variant (1, 4, 5, 6)
     float nmuasg[4 * kMaxTissues];
     int32_t ntissue, fetal_id;
@@ -154,7 +156,7 @@ struct KArgs {
 enum MiscSlot {  // slots of the misc tally (kMiscN int64 words)
     kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
     kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscDistQueries,
-    kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscUsed, kMiscN = 32
+    kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds, kMiscUsed, kMiscN = 32
 };
 
 static_assert(kMiscUsed <= kMiscN, "misc tally too small for its counters");

@@ -185,6 +185,14 @@ struct Hit {
 // Nearest face along p + t v with 0 < t < 10000 and t <= tmax, lowest
 // original index among equal t, ref excluded: the argmin FindTriangleInPath
 // (.cl:1037-1093) computes, restricted to t <= tmax.
+//
+// The running best is one 64-bit key (t bits, face id): t > 0, so the bits
+// of t order as t does, and the ordered tie rule of the reference loop --
+// accept t < best, or t == best with a lower id, the initial (tmax, none)
+// counting as id +inf -- is the single comparison key < best.  Written as
+// three separate conditional updates (t, id, med), hipcc (ROCm 7.2, gfx950)
+// kept the previous med when a face was taken through the t == best branch
+// (a hit at exactly t = tmax), so the key form is also what keeps med right.
 __device__ __forceinline__ void find_face(const KArgs& a, const float4* lnodes, f3 p, f3 v, int32_t ref,
                                           float tmax, Hit& h, uint32_t& visits, uint32_t& tests)
 {
@@ -197,6 +205,8 @@ __device__ __forceinline__ void find_face(const KArgs& a, const float4* lnodes, 
     if (!(__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z) &&
           __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z)))
         return;
+    uint64_t best = ((uint64_t)__float_as_uint(tmax) << 32) | 0xffffffffu;
+    int32_t med = 0;
     const float ix = (v.x != 0.0f) ? 1.0f / v.x : 3.0e38f;
     const float iy = (v.y != 0.0f) ? 1.0f / v.y : 3.0e38f;
     const float iz = (v.z != 0.0f) ? 1.0f / v.z : 3.0e38f;
@@ -212,11 +222,12 @@ __device__ __forceinline__ void find_face(const KArgs& a, const float4* lnodes, 
             hi = gn[2 * n + 1];
         }
         visits++;
+        const float bt = __uint_as_float((uint32_t)(best >> 32));
         float tx0 = (lo.x - p.x) * ix, tx1 = (hi.x - p.x) * ix;
         float ty0 = (lo.y - p.y) * iy, ty1 = (hi.y - p.y) * iy;
         float tz0 = (lo.z - p.z) * iz, tz1 = (hi.z - p.z) * iz;
         float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), h.t));
+        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), bt));
         int32_t hit = __float_as_int(lo.w), miss = __float_as_int(hi.w);
         if (tn <= tf) {
             if (hit >= 0) {
@@ -243,15 +254,19 @@ __device__ __forceinline__ void find_face(const KArgs& a, const float4* lnodes, 
                 if (!(ty == 0.0f || (ty > 0.0f) == wp || __builtin_fabsf(ty) < tiny)) continue;
                 if (!(tz == 0.0f || (tz > 0.0f) == wp || __builtin_fabsf(tz) < tiny)) continue;
                 float t = tx / w, k = ty / w, l = tz / w;
-                if (t > 0.0f && k >= 0.0f && l >= 0.0f && (k + l) < 1.0f && t < 10000.0f &&
-                    (t < h.t || (t == h.t && (h.id < 0 || id < h.id)))) {
-                    h.t = t;
-                    h.id = id;
-                    h.med = __float_as_int(B.w);
+                if (t > 0.0f && k >= 0.0f && l >= 0.0f && (k + l) < 1.0f && t < 10000.0f) {
+                    const uint64_t key = ((uint64_t)__float_as_uint(t) << 32) | (uint32_t)id;
+                    med = key < best ? __float_as_int(B.w) : med;
+                    best = key < best ? key : best;
                 }
             }
         }
         n = miss;
+    }
+    if ((uint32_t)best != 0xffffffffu) {
+        h.t = __uint_as_float((uint32_t)(best >> 32));
+        h.id = (int32_t)(uint32_t)best;
+        h.med = med;
     }
 }
 
@@ -356,59 +371,147 @@ __device__ __forceinline__ int64_t near_cell(const KArgs& a, f3 p)
     return ((int64_t)iz * a.ng_n[1] + iy) * a.ng_n[0] + ix;
 }
 
-// One pass over the near-face list of p's cell: the nearest face along
-// p + t v with t <= tmax (tmax <= near_r0; the exact test and tie rule of
-// find_face) and, when want_dist, the squared distances from p to the
-// nearest (d1sq, list entry i1) and second-nearest (d2sq) listed faces (+inf
-// when missing); unlisted = a lower bound on the distance to every face not
-// in the list.
-__device__ __forceinline__ void near_query(const KArgs& a, f3 p, f3 v, int32_t ref, float tmax, bool want_dist,
-                                           Hit& h, float& d1sq, float& d2sq, int32_t& i1, float& unlisted,
-                                           uint32_t& tests)
+// Near-face list query, wave-cooperative, for the lanes with `part` set: the
+// (photon, listed face) pairs of all those lanes are numbered by a prefix sum
+// over their list lengths and tested 64 at a time, one pair per lane, so a
+// pass costs one face test whatever the lengths of the individual lists.
+// Each pair's outcome is merged into its photon's result by a segmented
+// reduction (pairs of one photon are consecutive): the nearest hit is the
+// lexicographic minimum of (t, face id) over the hits with t <= tmax (the
+// nearest face and tie rule of find_face) and, when want, the two smallest
+// distances from p to the listed faces with the record of the smaller;
+// unlisted = a lower bound on the distance to every face not in the list.
+// Must be called by the whole wavefront.
+__device__ __forceinline__ void near_batch(const KArgs& a, bool part, f3 p, f3 v, int32_t ref, float tmax, bool want,
+                                        Hit& h, float& d1sq, float& d2sq, int32_t& i1, float& unlisted,
+                                        uint32_t& tests)
 {
+    const int lane = threadIdx.x & 63;
     h.t = tmax;
     h.id = -1;
     h.med = 0;
     d1sq = d2sq = __builtin_inff();
     i1 = -1;
     unlisted = a.near_r0;  // the grid reaches r0 beyond every face
-    int64_t c = near_cell(a, p);
-    if (c < 0) return;
-    const CellRec cr = a.near_cells[c];
-    const uint32_t b = cr.first, e = a.near_cells[c + 1].first;
-    unlisted = fmaxf(a.near_r0, __uint_as_float(cr.clear_bits));
+    uint32_t b = 0, n = 0;
+    if (part) {
+        int64_t c = near_cell(a, p);
+        if (c >= 0) {
+            const CellRec cr = a.near_cells[c];
+            b = cr.first;
+            n = a.near_cells[c + 1].first - b;
+            unlisted = fmaxf(a.near_r0, __uint_as_float(cr.clear_bits));
+        }
+    }
+    tests += n;
+    uint32_t incl = n;  // inclusive prefix sum of the list lengths
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+    }
+    const uint32_t excl = incl - n;
+    const uint32_t total = __shfl(incl, 63);
+    uint64_t best = ~0ull;  // (t bits, face id) of the nearest hit so far
+    int32_t best_med = 0;
     const float4* tr = reinterpret_cast<const float4*>(a.near_tris);
-    for (uint32_t k = b; k < e; ++k) {
-        float4 A = tr[3 * (int64_t)k], B = tr[3 * (int64_t)k + 1], C = tr[3 * (int64_t)k + 2];
-        tests++;
-        f3 M = mk(A.x, A.y, A.z), E = mk(B.x, B.y, B.z), F = mk(C.x, C.y, C.z);
-        if (want_dist) {
-            float dd = point_tri_dist2(p, M, E, F);
-            bool nearer = dd < d1sq;
-            d2sq = nearer ? d1sq : fminf(d2sq, dd);
-            i1 = nearer ? (int32_t)k : i1;
-            d1sq = nearer ? dd : d1sq;
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t g = base + lane;
+        // owner of pair g: the last lane whose first pair is <= g
+        int j = 0;
+#pragma unroll
+        for (int st = 32; st >= 1; st >>= 1) {
+            uint32_t ex = __shfl(excl, j + st);
+            if (ex <= g) j += st;
         }
-        int32_t id = __float_as_int(A.w);
-        if (id == ref) continue;
-        f3 G = sub(p, M);
-        f3 GxE = cross(G, E), vxF = cross(v, F);
-        float w = dot(vxF, E);
-        if (w == 0.0f) continue;
-        float tx = dot(GxE, F);
-        bool wp = w > 0.0f;
-        if ((tx > 0.0f) != wp) continue;
-        float ty = dot(vxF, G), tz = dot(GxE, v);
-        float tiny = __builtin_fabsf(w) * 1e-30f;
-        if (!(ty == 0.0f || (ty > 0.0f) == wp || __builtin_fabsf(ty) < tiny)) continue;
-        if (!(tz == 0.0f || (tz > 0.0f) == wp || __builtin_fabsf(tz) < tiny)) continue;
-        float t = tx / w, kk = ty / w, l = tz / w;
-        if (t > 0.0f && kk >= 0.0f && l >= 0.0f && (kk + l) < 1.0f && t < 10000.0f &&
-            (t < h.t || (t == h.t && (h.id < 0 || id < h.id)))) {
-            h.t = t;
-            h.id = id;
-            h.med = __float_as_int(B.w);
+        const f3 op = mk(__shfl(p.x, j), __shfl(p.y, j), __shfl(p.z, j));
+        const f3 ov = mk(__shfl(v.x, j), __shfl(v.y, j), __shfl(v.z, j));
+        const int32_t oref = __shfl(ref, j);
+        const float otmax = __shfl(tmax, j);
+        const int owant = __shfl((int)want, j);
+        const uint32_t k = __shfl(b, j) + (g - __shfl(excl, j));
+        uint64_t key = ~0ull;
+        int32_t med = 0;
+        float dd = __builtin_inff();
+        if (g < total) {
+            const float4 A = tr[3 * (int64_t)k], B = tr[3 * (int64_t)k + 1], C = tr[3 * (int64_t)k + 2];
+            const f3 M = mk(A.x, A.y, A.z), E = mk(B.x, B.y, B.z), F = mk(C.x, C.y, C.z);
+            if (owant) dd = point_tri_dist2(op, M, E, F);
+            const int32_t id = __float_as_int(A.w);
+            const f3 G = sub(op, M);
+            const f3 GxE = cross(G, E), vxF = cross(ov, F);
+            const float w = dot(vxF, E);
+            const float tx = dot(GxE, F);
+            const bool wp = w > 0.0f;
+            const float ty = dot(vxF, G), tz = dot(GxE, ov);
+            const float tiny = __builtin_fabsf(w) * 1e-30f;
+            bool ok = id != oref && w != 0.0f && (tx > 0.0f) == wp;
+            ok = ok && (ty == 0.0f || (ty > 0.0f) == wp || __builtin_fabsf(ty) < tiny);
+            ok = ok && (tz == 0.0f || (tz > 0.0f) == wp || __builtin_fabsf(tz) < tiny);
+            if (ok) {
+                const float t = tx / w, kk = ty / w, l = tz / w;
+                if (t > 0.0f && kk >= 0.0f && l >= 0.0f && (kk + l) < 1.0f && t < 10000.0f && t <= otmax) {
+                    key = ((uint64_t)__float_as_uint(t) << 32) | (uint32_t)id;
+                    med = __float_as_int(B.w);
+                }
+            }
         }
+        // segmented reduction towards the first pair of each photon in this pass
+        float e1 = dd, e2 = __builtin_inff();
+        int32_t ek = (int32_t)k;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int src = lane + off < 64 ? lane + off : lane;
+            const int sj = __shfl(j, src);
+            const uint32_t klo = __shfl((uint32_t)key, src), khi = __shfl((uint32_t)(key >> 32), src);
+            const int32_t smed = __shfl(med, src);
+            const float s1 = __shfl(e1, src), s2 = __shfl(e2, src);
+            const int32_t sk = __shfl(ek, src);
+            if (src != lane && sj == j) {
+                const uint64_t skey = ((uint64_t)khi << 32) | klo;
+                if (skey < key) {
+                    key = skey;
+                    med = smed;
+                }
+                if (s1 < e1) {
+                    e2 = fminf(e1, s2);
+                    e1 = s1;
+                    ek = sk;
+                } else {
+                    e2 = fminf(e2, s1);
+                }
+            }
+        }
+        // each owner takes the result at its first pair in this pass
+        const bool mine = n > 0 && excl < base + 64 && excl + n > base;
+        const int head = mine ? (int)((excl > base ? excl : base) - base) : lane;
+        const uint32_t rlo = __shfl((uint32_t)key, head), rhi = __shfl((uint32_t)(key >> 32), head);
+        const int32_t rmed = __shfl(med, head);
+        const float r1 = __shfl(e1, head), r2 = __shfl(e2, head);
+        const int32_t rk = __shfl(ek, head);
+        if (mine) {
+            const uint64_t rkey = ((uint64_t)rhi << 32) | rlo;
+            if (rkey < best) {
+                best = rkey;
+                best_med = rmed;
+            }
+            if (r1 < d1sq) {
+                d2sq = fminf(d1sq, r2);
+                d1sq = r1;
+                i1 = rk;
+            } else {
+                d2sq = fminf(d2sq, r1);
+            }
+        }
+    }
+    if (best != ~0ull) {
+        h.t = __uint_as_float((uint32_t)(best >> 32));
+        h.id = (int32_t)(uint32_t)best;
+        h.med = best_med;
+    }
+    if (!want) {
+        d1sq = d2sq = __builtin_inff();
+        i1 = -1;
     }
 }
 
@@ -473,7 +576,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     // events, escaped, capped, skipped searches, distance queries, NaN
     // photons, BVH visits, triangle tests, distance-query visits); flushed
     // to the 64-bit totals when one grows large and at the end
-    enum { kPhot, kSteps, kInt, kBnd, kEsc, kCap, kSkip, kDq, kNan, kVis, kTst, kDvis, kLst, kIterA, kRounds, kNc };
+    enum { kPhot, kSteps, kInt, kBnd, kEsc, kCap, kSkip, kDq, kNan, kVis, kTst, kDvis, kLst, kIterA, kRounds, kBvhRounds, kNc };
     uint32_t cnt[kNc];
 #pragma unroll
     for (int i = 0; i < kNc; ++i) cnt[i] = 0;
@@ -486,6 +589,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     bool ballistic = false;
     bool have_step = false;  // a step length has been drawn and not yet used
     bool search = false;     // the pending step needs a face search
+    bool by_list = false;    // ... from the near-face lists (else from the BVH)
 
     // interaction branch (.cl:1513-1537)
     auto interaction = [&](float s) {
@@ -777,7 +881,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         if (big && a.misc) {  // rare: keep the 32-bit counters from wrapping
             const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
                                    kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
-                                   kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds};
+                                   kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds};
 #pragma unroll
             for (int i = 0; i < kNc; ++i) {
                 atomic_add_u64(&a.misc[slot[i]], cnt[i]);
@@ -843,11 +947,14 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
 
         // ---- one round ----
         // Each lane draws its step length (.cl:1405-1442) when none is
-        // pending.  A step shorter than the lane's free radius needs no face
-        // search and is taken at once; a lane whose step may reach a face
-        // waits.  The search phases run once per round, for all waiting lanes
-        // together, after up to free_iters search-free iterations (or as soon
-        // as no lane can move without a search).
+        // pending.  A step whose end point stays in the lane's free region
+        // needs no face search and is taken at once; any other step waits for
+        // a search.  Waiting lanes are served in batches, by kind: short steps
+        // by the near-face lists, long ones (and lanes with no list) by the
+        // BVH.  A kind's batch runs once enough live lanes wait for it, or
+        // when no lane can move otherwise, or at the iteration cap that ends
+        // the round; a round also ends once enough lanes have finished, so
+        // that they are refilled.
         for (int it = 0;; ++it) {
             bool go = false;     // this lane takes a step in this iteration
             bool freed = false;  // ... and the step was shown to stay in the free region
@@ -884,23 +991,74 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     cnt[kSkip]++;
                 } else {
                     search = true;
+                    const bool finite_p =
+                        __builtin_isfinite(ph.p.x) && __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
+                    by_list = a.near_cells && !ballistic && stepsize <= a.near_r0 && finite_p;
                 }
             }
-            // start the search phase when enough of the live lanes wait for
-            // one (or none can move, or the iteration cap is reached)
-            const int nwait = __popcll(__ballot(search));
             const int nlive = __popcll(__ballot(alive));
-            const bool search_round = it + 1 >= a.free_iters || !__any(go) || nwait >= a.search_min * nlive;
+            const int nlist = __popcll(__ballot(search && by_list));
+            const int nbvh = __popcll(__ballot(search && !by_list));
+            const bool stuck = !__any(go);
+            const bool at_cap = it + 1 >= a.free_iters;
+            const bool do_list = nlist > 0 && (stuck || at_cap || nlist >= a.list_min * nlive);
+            const bool do_bvh = nbvh > 0 && (stuck || at_cap || nbvh >= a.search_min * nlive);
             if (lane == 0) cnt[kIterA]++;
-            if (search_round) {
+            if (do_list) {
                 if (lane == 0) cnt[kRounds]++;
-                // phase B: refresh the free radius of waiting lanes -- the
-                // nearest-face distance less a rounding margin (a face found by
-                // the exact test at t <= s lies within s + margin of p)
+                // exact answer from the cells' lists, and the free region with it
+                const bool part = search && by_list;
+                float d1sq, d2sq, unlisted;
+                int32_t i1;
+                const bool want = ph.ref < 0;
+                Hit hl;
+                near_batch(a, part, ph.p, ph.v, ph.ref, stepsize, want, hl, d1sq, d2sq, i1, unlisted, cnt[kLst]);
+                if (part) {
+                    h = hl;
+                    if (want) {
+                        const float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
+                                                __builtin_fabsf(ph.p.z));
+                        const float margin = a.safe_margin + 1e-5f * (a.near_r0 + rad);
+                        // cut by the nearest face's plane when p is clearly on
+                        // one side of it: the ball may then reach the
+                        // second-nearest face
+                        float d = fminf(__builtin_sqrtf(d1sq), unlisted);
+                        ph.pu = mk(0.0f, 0.0f, 0.0f);
+                        ph.pc = -1.0f;
+                        if (i1 >= 0 && d1sq < unlisted * unlisted) {
+                            const float4* tr = reinterpret_cast<const float4*>(a.near_tris);
+                            const float4 A = tr[3 * (int64_t)i1], B = tr[3 * (int64_t)i1 + 1],
+                                         C = tr[3 * (int64_t)i1 + 2];
+                            const f3 n = cross(mk(B.x, B.y, B.z), mk(C.x, C.y, C.z));
+                            const f3 u = mul(1.0f / __builtin_sqrtf(dot(n, n)), n);
+                            const float c0 = dot(u, mk(A.x, A.y, A.z));
+                            const float sd = dot(u, ph.p) - c0;
+                            if (__builtin_fabsf(sd) > 4.0f * margin) {  // false for a degenerate face (NaN)
+                                const float sg = sd > 0.0f ? 1.0f : -1.0f;
+                                ph.pu = mul(sg, u);
+                                ph.pc = sg * c0 + margin;
+                                d = fminf(__builtin_sqrtf(d2sq), unlisted);
+                            }
+                        }
+                        ph.safe = fmaxf(0.0f, d - (a.safe_margin + 1e-5f * (d + rad)));  // >= 0: the test squares it
+                        ph.anc = ph.p;
+                        cnt[kDq]++;
+                    }
+                    bnd = h.id >= 0;
+                    if (h.id < 0) h.t = 10000.0f;
+                    search = false;
+                    go = true;
+                }
+            }
+            if (do_bvh) {
+                if (lane == 0) cnt[kBvhRounds]++;
+                const bool sb = search && !by_list;
+                // refresh the free radius of waiting lanes -- the nearest-face
+                // distance less a rounding margin (a face found by the exact
+                // test at t <= s lies within s + margin of p)
                 const bool finite_p =
                     __builtin_isfinite(ph.p.x) && __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
-                const bool use_list = a.near_cells && !ballistic && stepsize <= a.near_r0 && finite_p;
-                bool dq = search && !use_list && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && finite_p;
+                bool dq = sb && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && finite_p;
                 if (__any(dq)) {
                     if (dq) {
                         float cap = fminf(a.dist_cap, a.dist_cap_steps * stepsize);
@@ -919,54 +1077,15 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                         }
                     }
                 }
-                // phase C: nearest face within the pending step
-                if (__any(search)) {
-                    if (search) {
-                        if (use_list) {
-                            // exact answer from the cell's list, and the free radius with it
-                            float d1sq, d2sq, unlisted;
-                            int32_t i1;
-                            bool want = ph.ref < 0;
-                            near_query(a, ph.p, ph.v, ph.ref, stepsize, want, h, d1sq, d2sq, i1, unlisted, cnt[kLst]);
-                            if (want) {
-                                const float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
-                                                        __builtin_fabsf(ph.p.z));
-                                const float margin = a.safe_margin + 1e-5f * (a.near_r0 + rad);
-                                // cut by the nearest face's plane when p is
-                                // clearly on one side of it: the ball may then
-                                // reach the second-nearest face
-                                float d = fminf(__builtin_sqrtf(d1sq), unlisted);
-                                ph.pu = mk(0.0f, 0.0f, 0.0f);
-                                ph.pc = -1.0f;
-                                if (i1 >= 0 && d1sq < unlisted * unlisted) {
-                                    const float4* tr = reinterpret_cast<const float4*>(a.near_tris);
-                                    const float4 A = tr[3 * (int64_t)i1], B = tr[3 * (int64_t)i1 + 1],
-                                                 C = tr[3 * (int64_t)i1 + 2];
-                                    const f3 n = cross(mk(B.x, B.y, B.z), mk(C.x, C.y, C.z));
-                                    const f3 u = mul(1.0f / __builtin_sqrtf(dot(n, n)), n);
-                                    const float c0 = dot(u, mk(A.x, A.y, A.z));
-                                    const float sd = dot(u, ph.p) - c0;
-                                    if (__builtin_fabsf(sd) > 4.0f * margin) {  // false for a degenerate face (NaN)
-                                        const float sg = sd > 0.0f ? 1.0f : -1.0f;
-                                        ph.pu = mul(sg, u);
-                                        ph.pc = sg * c0 + margin;
-                                        d = fminf(__builtin_sqrtf(d2sq), unlisted);
-                                    }
-                                }
-                                ph.safe = fmaxf(0.0f, d - (a.safe_margin + 1e-5f * (d + rad)));  // >= 0: the test squares it
-                                ph.anc = ph.p;
-                                cnt[kDq]++;
-                            }
-                        } else {
-                            find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, cnt[kVis],
-                                      cnt[kTst]);
-                        }
-                        bnd = h.id >= 0 || ballistic || stepsize >= 10000.0f;  // no face ahead at all: escape
-                        if (h.id < 0) h.t = 10000.0f;
-                        if (ballistic) stepsize = h.t;
-                        search = false;
-                        go = true;
-                    }
+                // nearest face within the pending step
+                if (search && !by_list) {
+                    find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, cnt[kVis],
+                              cnt[kTst]);
+                    bnd = h.id >= 0 || ballistic || stepsize >= 10000.0f;  // no face ahead at all: escape
+                    if (h.id < 0) h.t = 10000.0f;
+                    if (ballistic) stepsize = h.t;
+                    search = false;
+                    go = true;
                 }
             }
             // ---- the step itself (the reference loop body, .cl:1375-1575) ----
@@ -988,13 +1107,15 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     alive = false;
                 }
             }
-            if (search_round) break;
+            // end the round at the cap, or once enough lanes have finished
+            // and photons are left to start
+            if (at_cap || (!exhausted && 64 - __popcll(__ballot(alive)) >= a.refill_min)) break;
         }
     }
     if (a.misc) {
         const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
                                kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
-                               kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds};
+                               kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds};
 #pragma unroll
         for (int i = 0; i < kNc; ++i) {
             uint64_t v = cnt[i];

@@ -384,7 +384,7 @@ __device__ __forceinline__ int64_t near_cell(const KArgs& a, f3 p)
 // Must be called by the whole wavefront.
 __device__ __forceinline__ void near_batch(const KArgs& a, bool part, f3 p, f3 v, int32_t ref, float tmax, bool want,
                                         Hit& h, float& d1sq, float& d2sq, int32_t& i1, float& unlisted,
-                                        uint32_t& tests)
+                                        uint32_t& pairs)
 {
     const int lane = threadIdx.x & 63;
     h.t = tmax;
@@ -403,7 +403,6 @@ __device__ __forceinline__ void near_batch(const KArgs& a, bool part, f3 p, f3 v
             unlisted = fmaxf(a.near_r0, __uint_as_float(cr.clear_bits));
         }
     }
-    tests += n;
     uint32_t incl = n;  // inclusive prefix sum of the list lengths
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -412,6 +411,7 @@ __device__ __forceinline__ void near_batch(const KArgs& a, bool part, f3 p, f3 v
     }
     const uint32_t excl = incl - n;
     const uint32_t total = __shfl(incl, 63);
+    pairs = total;
     uint64_t best = ~0ull;  // (t bits, face id) of the nearest hit so far
     int32_t best_med = 0;
     const float4* tr = reinterpret_cast<const float4*>(a.near_tris);
@@ -538,12 +538,16 @@ struct Photon {
     int64_t k;  // index within this run
 };
 
+// per-wavefront event counters in LDS
+enum WaveCount { wPhot, wInt, wBnd, wEsc, wCap, wSkip, wDq, wNan, wLst, wIter, wLRounds, wBRounds, wArzOut, wVoxOut, kWc };
+
 // WAVES: minimum waves per SIMD the register allocator must allow (1 =
 // the compiler's choice); a tuning knob, results do not depend on it.
 template <int RNG, bool TRACE, int WAVES>
 __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
 {
     extern __shared__ float4 smem[];
+    __shared__ unsigned long long wcnt[kBlock / 64][kWc];
     float4* lnodes = smem;                         // 2 * nlds
     float4* lmat = smem + 2 * a.nlds;              // ntissue
     int32_t* lran3 = reinterpret_cast<int32_t*>(lmat + kMaxTissues);
@@ -553,6 +557,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     {
         const float4* gn = reinterpret_cast<const float4*>(a.nodes);
         for (int i = tid; i < 2 * a.nlds; i += kBlock) lnodes[i] = gn[i];
+        for (int i = tid; i < (kBlock / 64) * kWc; i += kBlock) (&wcnt[0][0])[i] = 0;
         for (int i = tid; i < a.ntissue; i += kBlock)
             lmat[i] = make_float4(a.nmuasg[4 * i], a.nmuasg[4 * i + 1], a.nmuasg[4 * i + 2], a.nmuasg[4 * i + 3]);
     }
@@ -576,11 +581,17 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     // events, escaped, capped, skipped searches, distance queries, NaN
     // photons, BVH visits, triangle tests, distance-query visits); flushed
     // to the 64-bit totals when one grows large and at the end
-    enum { kPhot, kSteps, kInt, kBnd, kEsc, kCap, kSkip, kDq, kNan, kVis, kTst, kDvis, kLst, kIterA, kRounds, kBvhRounds, kNc };
-    uint32_t cnt[kNc];
-#pragma unroll
-    for (int i = 0; i < kNc; ++i) cnt[i] = 0;
-    uint64_t c_arz_out = 0, c_vox_out = 0;
+    // Event counters live in LDS, one row per wavefront: the first active
+    // lane adds the number of active lanes (or a lane adds its own rare
+    // weight), so counting costs no registers.  Only the visit counts of
+    // the BVH loops are kept per lane (flushed when one grows large and at
+    // the end).
+    unsigned long long* wc = wcnt[tid >> 6];
+    auto wcount = [&](int slot) {
+        const unsigned long long m = __ballot(1);
+        if (lane == __ffsll((long long)m) - 1) atomicAdd(&wc[slot], (unsigned long long)__popcll(m));
+    };
+    uint32_t lvis = 0, ltst = 0, ldvis = 0;  // BVH node visits, triangle tests, distance-query visits
     unsigned long long reg_acc = 0;  // fixed-point absorption in ph.curr not yet flushed
     int64_t vox_cell = -1;           // voxel whose deposits are pending in vox_acc
     unsigned long long vox_acc = 0;
@@ -654,7 +665,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 atomic_add_u64(&a.arz[cell], q);
                 atomic_add_u64(&a.phirz[cell], fx24(ph.absorbed / (cm.y == 0.0f ? 1.0f : cm.y)));
             } else {
-                c_arz_out += q;
+                if (q) atomicAdd(&wc[wArzOut], q);
             }
         }
         if (a.vox) {
@@ -671,9 +682,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             if (ok && !same && vox_cell >= 0) atomic_add_u64(&a.vox[vox_cell], vox_acc);
             vox_acc = same ? vox_acc + q : (ok ? q : vox_acc);
             vox_cell = ok ? cell : vox_cell;
-            c_vox_out += ok ? 0ull : (unsigned long long)q;
+            if (!ok && q) atomicAdd(&wc[wVoxOut], q);
         }
-        cnt[kInt]++;
+        wcount(wInt);
     };
 
     // boundary branch (.cl:1462-1511) at face h.id, distance stb
@@ -686,7 +697,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         if (h.id < 0) {  // nothing ahead: the photon has left the mesh
             kind = 3;
             ph.w = 0.0f;
-            cnt[kEsc]++;
+            wcount(wEsc);
         } else {
             float x = rng.next();
             int32_t in_id = h.med & 0xffff, out_id = (h.med >> 16) & 0xffff;
@@ -768,7 +779,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             if (ph.curr == a.fetal_id) ph.fetal = 1;
             ph.unfin = (s - stb) * (cm.y + cm.z);
         }
-        cnt[kBnd]++;
+        wcount(wBnd);
         return kind;
     };
 
@@ -786,7 +797,6 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             }
         }
         ph.steps++;
-        cnt[kSteps]++;
         if (TRACE && ph.steps <= a.trace_max) {
             float* t = a.trace + ((int64_t)ph.k * a.trace_max + (ph.steps - 1)) * 12;
             t[0] = (float)kind;
@@ -803,7 +813,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             t[11] = (float)(TRACE ? rng.ndraw : 0);
         }
         if (a.max_steps > 0 && ph.steps >= a.max_steps && ph.w > 0.0f) {
-            cnt[kCap]++;
+            wcount(wCap);
             ph.w = 0.0f;
         }
         return ph.w > 0.0f && (ph.w > a.w_th || ph.roulette);
@@ -813,7 +823,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     auto epilogue = [&]() {
         if (ph.p.x != ph.p.x || ph.p.y != ph.p.y || ph.p.z != ph.p.z || ph.v.x != ph.v.x || ph.v.y != ph.v.y ||
             ph.v.z != ph.v.z)
-            cnt[kNan]++;
+            wcount(wNan);
         if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
         reg_acc = 0;
         if (vox_cell >= 0) atomic_add_u64(&a.vox[vox_cell], vox_acc);
@@ -875,18 +885,11 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 }
             }
         }
-        bool big = false;
-#pragma unroll
-        for (int i = 0; i < kNc; ++i) big |= cnt[i] > 0x40000000u;
-        if (big && a.misc) {  // rare: keep the 32-bit counters from wrapping
-            const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
-                                   kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
-                                   kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds};
-#pragma unroll
-            for (int i = 0; i < kNc; ++i) {
-                atomic_add_u64(&a.misc[slot[i]], cnt[i]);
-                cnt[i] = 0;
-            }
+        if ((lvis | ltst | ldvis) > 0x40000000u && a.misc) {  // rare: keep the 32-bit counts from wrapping
+            atomic_add_u64(&a.misc[kMiscNodeVisits], lvis);
+            atomic_add_u64(&a.misc[kMiscTriTests], ltst);
+            atomic_add_u64(&a.misc[kMiscDistVisits], ldvis);
+            lvis = ltst = ldvis = 0;
         }
     };
 
@@ -938,7 +941,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                         alive = true;
                         have_step = false;
                         search = false;
-                        cnt[kPhot]++;
+                        wcount(wPhot);
                     }
                 }
             }
@@ -988,7 +991,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 freed = !ballistic && dot(e, e) < ph.safe * ph.safe && dot(ph.pu, q) > ph.pc;
                 if (freed || (!ballistic && a.safety && stepsize < safety_at(a, ph.p))) {
                     go = true;
-                    cnt[kSkip]++;
+                    wcount(wSkip);
                 } else {
                     search = true;
                     const bool finite_p =
@@ -1003,16 +1006,18 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             const bool at_cap = it + 1 >= a.free_iters;
             const bool do_list = nlist > 0 && (stuck || at_cap || nlist >= a.list_min * nlive);
             const bool do_bvh = nbvh > 0 && (stuck || at_cap || nbvh >= a.search_min * nlive);
-            if (lane == 0) cnt[kIterA]++;
+            if (lane == 0) atomicAdd(&wc[wIter], 1ull);
             if (do_list) {
-                if (lane == 0) cnt[kRounds]++;
+                if (lane == 0) atomicAdd(&wc[wLRounds], 1ull);
                 // exact answer from the cells' lists, and the free region with it
                 const bool part = search && by_list;
                 float d1sq, d2sq, unlisted;
                 int32_t i1;
                 const bool want = ph.ref < 0;
                 Hit hl;
-                near_batch(a, part, ph.p, ph.v, ph.ref, stepsize, want, hl, d1sq, d2sq, i1, unlisted, cnt[kLst]);
+                uint32_t pairs;
+                near_batch(a, part, ph.p, ph.v, ph.ref, stepsize, want, hl, d1sq, d2sq, i1, unlisted, pairs);
+                if (lane == 0) atomicAdd(&wc[wLst], (unsigned long long)pairs);
                 if (part) {
                     h = hl;
                     if (want) {
@@ -1042,7 +1047,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                         }
                         ph.safe = fmaxf(0.0f, d - (a.safe_margin + 1e-5f * (d + rad)));  // >= 0: the test squares it
                         ph.anc = ph.p;
-                        cnt[kDq]++;
+                        wcount(wDq);
                     }
                     bnd = h.id >= 0;
                     if (h.id < 0) h.t = 10000.0f;
@@ -1051,7 +1056,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 }
             }
             if (do_bvh) {
-                if (lane == 0) cnt[kBvhRounds]++;
+                if (lane == 0) atomicAdd(&wc[wBRounds], 1ull);
                 const bool sb = search && !by_list;
                 // refresh the free radius of waiting lanes -- the nearest-face
                 // distance less a rounding margin (a face found by the exact
@@ -1062,25 +1067,24 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 if (__any(dq)) {
                     if (dq) {
                         float cap = fminf(a.dist_cap, a.dist_cap_steps * stepsize);
-                        float d = mesh_distance(a, lnodes, ph.p, cap, cnt[kDvis]);
+                        float d = mesh_distance(a, lnodes, ph.p, cap, ldvis);
                         float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
                                           __builtin_fabsf(ph.p.z));
                         ph.safe = fmaxf(0.0f, d - (a.safe_margin + 1e-5f * (d + rad)));  // >= 0: the test squares it
                         ph.anc = ph.p;
                         ph.pu = mk(0.0f, 0.0f, 0.0f);
                         ph.pc = -1.0f;
-                        cnt[kDq]++;
+                        wcount(wDq);
                         if (stepsize < ph.safe) {
                             search = false;
                             go = true;
-                            cnt[kSkip]++;
+                            wcount(wSkip);
                         }
                     }
                 }
                 // nearest face within the pending step
                 if (search && !by_list) {
-                    find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, cnt[kVis],
-                              cnt[kTst]);
+                    find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, lvis, ltst);
                     bnd = h.id >= 0 || ballistic || stepsize >= 10000.0f;  // no face ahead at all: escape
                     if (h.id < 0) h.t = 10000.0f;
                     if (ballistic) stepsize = h.t;
@@ -1113,22 +1117,31 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         }
     }
     if (a.misc) {
-        const int slot[kNc] = {kMiscPhotons, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
-                               kMiscSkipped, kMiscDistQueries, kMiscNan, kMiscNodeVisits, kMiscTriTests,
-                               kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds};
+        const uint32_t lane_cnt[3] = {lvis, ltst, ldvis};
+        const int lane_slot[3] = {kMiscNodeVisits, kMiscTriTests, kMiscDistVisits};
 #pragma unroll
-        for (int i = 0; i < kNc; ++i) {
-            uint64_t v = cnt[i];
+        for (int i = 0; i < 3; ++i) {
+            uint64_t v = lane_cnt[i];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) atomic_add_u64(&a.misc[slot[i]], v);
+            if (lane == 0) atomic_add_u64(&a.misc[lane_slot[i]], v);
         }
-        uint64_t w2[2] = {c_arz_out, c_vox_out};
-        const int slot2[2] = {kMiscArzOut, kMiscVoxOut};
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            uint64_t v = w2[i];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) atomic_add_u64(&a.misc[slot2[i]], v);
+        __threadfence_block();
+        if (lane == 0) {
+            atomic_add_u64(&a.misc[kMiscPhotons], wc[wPhot]);
+            atomic_add_u64(&a.misc[kMiscSteps], wc[wInt] + wc[wBnd]);  // every step is one or the other
+            atomic_add_u64(&a.misc[kMiscInteract], wc[wInt]);
+            atomic_add_u64(&a.misc[kMiscBoundary], wc[wBnd]);
+            atomic_add_u64(&a.misc[kMiscEscaped], wc[wEsc]);
+            atomic_add_u64(&a.misc[kMiscCapped], wc[wCap]);
+            atomic_add_u64(&a.misc[kMiscSkipped], wc[wSkip]);
+            atomic_add_u64(&a.misc[kMiscDistQueries], wc[wDq]);
+            atomic_add_u64(&a.misc[kMiscNan], wc[wNan]);
+            atomic_add_u64(&a.misc[kMiscListTests], wc[wLst]);
+            atomic_add_u64(&a.misc[kMiscIters], wc[wIter]);
+            atomic_add_u64(&a.misc[kMiscRounds], wc[wLRounds]);
+            atomic_add_u64(&a.misc[kMiscBvhRounds], wc[wBRounds]);
+            atomic_add_u64(&a.misc[kMiscArzOut], wc[wArzOut]);
+            atomic_add_u64(&a.misc[kMiscVoxOut], wc[wVoxOut]);
         }
     }
 }

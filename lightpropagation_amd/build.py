@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblp.so")
 
-SOURCES = ["lp_kernel.hip", "lp_capi.cpp", "lp_bvh.cpp", "lp_model.cpp", "lp_host_facade.cpp"]
+SOURCES = ["csrc/lp_kernel.hip", "csrc/lp_capi.cpp", "csrc/lp_bvh.cpp", "csrc/lp_model.cpp", "host/lp_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LP_OFFLOAD_ARCH", "gfx950")
@@ -52,10 +52,8 @@ def build(verbose=False, jobs=None):
     objs = []
     procs = []
     for src in SOURCES:
-        path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
-        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+        path = os.path.join(HERE, src)
+        obj = os.path.join(build_dir, os.path.splitext(os.path.basename(src))[0] + ".o")
         if src.endswith(".hip"):
             cmd = [HIPCC] + COMMON + DEVICE + ["-c", "-o", obj, path]
         else:

@@ -81,7 +81,8 @@ def parse_obj(path, tissue_names):
                 outer = int(re.match(r"^g (\d+)$", line).group(1))
             elif re.match(r"^g (\d+)_(\w+)", line):
                 outer = int(re.match(r"^g (\d+)_(\w+)", line).group(1))
-    V = np.array(verts, np.float64)
+    # vertices as float32 (vtkOBJReader's points), normals from them in double
+    V = np.array(verts, np.float64).astype(f32).astype(np.float64)
     F = np.array(faces, np.int64)
     P0, P1, P2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
     n = np.cross(P1 - P0, P2 - P0)

@@ -9,7 +9,11 @@ detectors (2-10 cm), a 256^3 int64 fixed-point absorption grid (0.05 cm
 voxels), per-region absorption, 200 radial bins, the SFT and RTd images.
 One step = one lp_run of --photons photons with inputs resident on the GPU.
 
-Prints one JSON line (see DESIGN.md for the roofline and baseline figures).
+Prints one JSON line: the metric, the FP32-VALU roofline of the transport
+kernel (algorithmic work per photon step over the launch time measured with
+HIP events), its memory traffic per launch (from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE summary of this workload, scaled per photon), and
+the CPU baseline (the oracle on a bounded sample of the same workload).
 """
 import argparse
 import json
@@ -31,6 +35,21 @@ METRIC = "photons/sec on fetal mesh (1 GPU); adjoint vs finite-diff gradient tim
 # triangle test (52 + 3) plus ceil(log2 T) box tests of 20.
 F_INT, F_BND, F_TRI, F_BOX = 225.0, 125.0, 55.0, 20.0
 FP32_PEAK_TFLOPS = 157.3
+PROFILE = os.path.join(ROOT, "profiles", "transport_kernel_maternal_fetal_202k.md")
+
+
+def traffic_per_photon():
+    """Memory traffic of the transport kernel per photon, from the committed
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
+    copy kernel there); None when the summary is absent."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_photon:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def detectors():
@@ -89,8 +108,9 @@ def main():
                     help="also run a plain copy of this many MiB (reference for profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print("bench.py measures one GPU: this library runs on a single MI355X; sharding photons over "
+              "several GPUs (disjoint photon ranges, one int64 all-reduce of the tallies) is not "
+              "implemented", file=sys.stderr)
         sys.exit(2)
 
     import lightpropagation_amd as lp
@@ -133,6 +153,7 @@ def main():
     search = F_TRI + F_BOX * math.ceil(math.log2(max(T, 2)))
     flops = m["interact"] * (F_INT + search) + m["boundary"] * (F_BND + search)
     achieved = flops / kernel_s / 1e12
+    tpp = traffic_per_photon()
     R = t["sft"].reshape(-1, 3).sum(0) / lp.FIXED / nph
     det = t["det"].reshape(-1, 2)[:, 0] / lp.FIXED / nph
     out = {
@@ -159,7 +180,9 @@ def main():
         "kernel_ms_per_step": sum(kms) / len(kms),
         "steps_per_photon": steps / nph,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": (tpp * args.photons) if tpp is not None else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)",
                      "work": "algorithmic fp32 per step: interaction %g + boundary %g, search %g (T=%d)" %
                              (F_INT, F_BND, search, T)},
         "counters": {k: m[k] for k in ("interact", "boundary", "escaped", "capped", "nan", "node_visits",

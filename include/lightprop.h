@@ -12,8 +12,9 @@
  * CalculatePropagation setters / Init() / Propagation() / Get*Image()
  * (CalculatePropagation.hpp:212-389; call order main.cpp:174-222).  Each
  * entry point below names the reference interface it replaces.  The C++
- * facade lightpropagation_amd/host/CalculatePropagation.hpp keeps the
- * reference's class interface on top of this ABI.
+ * host layer lightpropagation_amd/host/lp_host.hpp keeps the reference's
+ * class interfaces (CalculatePropagation and its neighbours) on top of this
+ * ABI; include/lightprop_host.h is that layer's own C ABI.
  */
 #ifndef LIGHTPROP_H
 #define LIGHTPROP_H

@@ -84,6 +84,10 @@ typedef struct lp_stats {
     int32_t bvh_depth;
     int32_t grid_blocks;    /* persistent work-groups launched                  */
     int64_t device_bytes;   /* device memory held by the context               */
+    double replay_ms;       /* lp_adjoint: device time of the replay launch     */
+    int64_t replayed;       /* lp_adjoint: (photon, detector) pairs replayed    */
+    int64_t replay_mismatch; /* lp_adjoint: replays that ended with another R
+                                weight than the forward photon (0 when exact)   */
 } lp_stats;
 
 /* Replaces the CalculatePropagation constructor + createContext /
@@ -157,6 +161,25 @@ int lp_get_exit_records(lp_context *ctx, float *out, int64_t nphotons);
 int lp_trace(lp_context *ctx, int64_t first, int32_t count, int32_t max_rec, float *out, int32_t *nrec);
 
 int lp_get_stats(lp_context *ctx, lp_stats *out);
+
+/* ---- adjoint (no reference counterpart) ----
+ * Gradients of the annular detector signals S_d (sum of the R exit weights
+ * W of the photons whose exit radius lies in detector d, as LP_TALLY_DET
+ * counts them) with respect to each tissue's mua and mus, and to mua in each
+ * voxel of a grid in the emitter frame, by the path-length (perturbation
+ * Monte Carlo) estimator on every detected path:
+ *   dW/dmua_t = -L_t W,   dW/dmus_t = (n_t / mus_t - L_t) W,   dW/dmua_v = -L_v W
+ * (L: path length, n: interactions in tissue t / voxel v).  lp_adjoint runs
+ * photons [first, first + count) forward (adding to the tallies as lp_run
+ * does) while recording which photons reach which detector, then replays
+ * only those photons from their (seed, index) random streams in a second
+ * launch that accumulates the gradients.  Outputs are 2^32 fixed point:
+ * signals[ndet], grad[ndet][ntissue][2] (d/dmua, d/dmus), jac[ndet][nvox]
+ * (x fastest; NULL skips the voxel Jacobian); npairs = (photon, detector)
+ * pairs replayed.  Needs detectors and at most 16 tissues. */
+int lp_set_jacobian_grid(lp_context *ctx, const float origin[3], const float size[3], const int32_t n[3]);
+int lp_adjoint(lp_context *ctx, int64_t first, int64_t count, int64_t *signals, int64_t *grad, int64_t *jac,
+               int64_t *npairs);
 
 /* Evaluates the kernel's math core element-wise on the device (op 0 sqrt,
  * 1 divide, 2 log, 3 cos, 4 acos, 5 multiply, 6 add) over n (x, y) pairs. */

@@ -36,7 +36,8 @@ class Layout(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("photons", C.c_int64), ("bvh_nodes", C.c_int64),
-                ("bvh_depth", C.c_int32), ("grid_blocks", C.c_int32), ("device_bytes", C.c_int64)]
+                ("bvh_depth", C.c_int32), ("grid_blocks", C.c_int32), ("device_bytes", C.c_int64),
+                ("replay_ms", C.c_double), ("replayed", C.c_int64), ("replay_mismatch", C.c_int64)]
 
 
 class LPError(RuntimeError):
@@ -65,6 +66,8 @@ _SIGS = {
     "lp_get_exit_records": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
     "lp_trace": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, _fp, _ip]),
     "lp_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "lp_set_jacobian_grid": (C.c_int, [C.c_void_p, _fp, _fp, _ip]),
+    "lp_adjoint": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _lp, _lp, _lp, _lp]),
     "lp_debug_math": (C.c_int, [C.c_void_p, C.c_int32, _fp, _fp, C.c_int64]),
     "lp_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "lp_set_option": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
@@ -147,6 +150,7 @@ class Context:
     def set_materials(self, nmuasg, fetal_id):
         nm = _f(nmuasg).reshape(-1)
         self._chk(lib().lp_set_materials(self._c, fp(nm), len(nm) // 4, int(fetal_id)), "lp_set_materials")
+        self._keep["ntissue"] = len(nm) // 4
 
     def set_source(self, pstart, vstart, vocc, poi_basis):
         ps, vs, vo, pb = _f(pstart), _f(vstart).reshape(-1), _i(vocc).reshape(-1), _f(poi_basis)
@@ -168,6 +172,7 @@ class Context:
     def set_detectors(self, rlo, rhi):
         lo, hi = _f(rlo), _f(rhi)
         self._chk(lib().lp_set_detectors(self._c, fp(lo), fp(hi), len(lo)), "lp_set_detectors")
+        self._keep["ndet"] = len(lo)
 
     def set_tallies(self, mask=0xFFFFFFFF, record_exit=False):
         self._chk(lib().lp_set_tallies(self._c, mask, int(bool(record_exit))), "lp_set_tallies")
@@ -223,6 +228,29 @@ class Context:
         nrec = np.zeros(count, np.int32)
         self._chk(lib().lp_trace(self._c, first, count, max_rec, fp(out), ip(nrec)), "lp_trace")
         return out, nrec
+
+    def set_jacobian_grid(self, origin, size, n):
+        self._jac_n = [int(x) for x in n]
+        self._chk(lib().lp_set_jacobian_grid(self._c, fp(_f(origin)), fp(_f(size)), ip(_i(n))),
+                  "lp_set_jacobian_grid")
+
+    def adjoint(self, first, count, jac=False, ntissue=None, ndet=None):
+        """lp_adjoint: dict(signals (ndet,), grad (ndet, ntissue, 2),
+        jac (ndet, nz, ny, nx) or None, pairs) as int64 fixed point
+        (2^32 per unit); the photons also add to the tallies."""
+        nt = ntissue if ntissue is not None else self._keep["ntissue"]
+        nd = ndet if ndet is not None else self._keep["ndet"]
+        sig = np.zeros(nd, np.int64)
+        grad = np.zeros((nd, nt, 2), np.int64)
+        J = None
+        if jac:
+            nx, ny, nz = getattr(self, "_jac_n", (0, 0, 0))
+            J = np.zeros((nd, nz, ny, nx), np.int64)
+        pairs = C.c_int64()
+        self._chk(lib().lp_adjoint(self._c, int(first), int(count), sig.ctypes.data_as(_lp),
+                                   grad.ctypes.data_as(_lp), J.ctypes.data_as(_lp) if J is not None else None,
+                                   C.byref(pairs)), "lp_adjoint")
+        return dict(signals=sig, grad=grad, jac=J, pairs=pairs.value)
 
     def stats(self):
         s = Stats()

@@ -64,6 +64,8 @@ struct lp_context {
     int32_t refill_min = 16;
     int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
+    float jac_origin[3] = {0, 0, 0}, jac_size[3] = {1, 1, 1};
+    int32_t jac_n[3] = {0, 0, 0};
 
     // derived
     lp_layout L{};
@@ -77,6 +79,7 @@ struct lp_context {
     // device
     DevBuf d_tris, d_nodes, d_nrm, d_med, d_safety, d_vstart, d_vocc, d_counter, d_exit;
     DevBuf d_tally[LP_NTALLY];
+    DevBuf d_det_list, d_det_count, d_grad, d_jac, d_adj_misc;
     int64_t exit_count = 0;
     lp_stats stats{};
 };
@@ -413,7 +416,7 @@ void lp_destroy(lp_context* c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
-                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris};
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list, &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_adj_misc};
     for (DevBuf* b : bufs) dev_free(*b);
     for (auto& b : c->d_tally) dev_free(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -708,12 +711,119 @@ int lp_trace(lp_context* c, int64_t first, int32_t count, int32_t max_rec, float
     return rc;
 }
 
+int lp_set_jacobian_grid(lp_context* c, const float origin[3], const float size[3], const int32_t n[3])
+{
+    if (!c) return LP_EINVAL;
+    if (!origin || !size || !n) return fail(c, LP_EINVAL, "lp_set_jacobian_grid: null argument");
+    int64_t nv = 1;
+    for (int k = 0; k < 3; ++k) {
+        if (n[k] < 0 || !(size[k] > 0)) return fail(c, LP_EINVAL, "lp_set_jacobian_grid: bad grid");
+        nv *= n[k];
+    }
+    if (nv > ((int64_t)1 << 28)) return fail(c, LP_EINVAL, "lp_set_jacobian_grid: at most 2^28 voxels");
+    for (int k = 0; k < 3; ++k) {
+        c->jac_origin[k] = origin[k];
+        c->jac_size[k] = size[k];
+        c->jac_n[k] = n[k];
+    }
+    return LP_OK;
+}
+
+int lp_adjoint(lp_context* c, int64_t first, int64_t count, int64_t* signals, int64_t* grad, int64_t* jac,
+               int64_t* npairs)
+{
+    if (!c) return LP_EINVAL;
+    if (first < 0 || count < 0) return fail(c, LP_EINVAL, "lp_adjoint: negative range");
+    int rc = prepare(c);
+    if (rc) return rc;
+    const int32_t nd = (int32_t)c->det_rlo.size(), nt = c->ntissue;
+    if (nd < 1) return fail(c, LP_ESTATE, "lp_adjoint: no detectors (lp_set_detectors)");
+    if (nt > kAdjMaxTissues) return fail(c, LP_EINVAL, "lp_adjoint: at most %d tissues", kAdjMaxTissues);
+    const int64_t nvox = (int64_t)c->jac_n[0] * c->jac_n[1] * c->jac_n[2];
+    // capacity: photons x the most detectors any exit radius can lie in
+    int32_t overlap = 0;
+    for (int32_t i = 0; i < nd; ++i) {
+        int32_t o = 0;
+        for (int32_t j = 0; j < nd; ++j)
+            if (c->det_rlo[j] <= c->det_rlo[i] && c->det_rlo[i] < c->det_rhi[j]) ++o;
+        overlap = std::max(overlap, o);
+    }
+    const int64_t cap = std::max<int64_t>(1, count * std::max(overlap, 1));
+
+    // 1. forward, recording the detected photons
+    if ((rc = dev_alloc(c, c->d_det_list, (size_t)cap * sizeof(DetRec)))) return rc;
+    if ((rc = dev_alloc(c, c->d_det_count, 64))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_det_count.p, 0, 8, c->stream));
+    KArgs a;
+    fill_args(c, a);
+    a.first = first;
+    a.count = count;
+    a.det_list = (DetRec*)c->d_det_list.p;
+    a.det_count = (unsigned long long*)c->d_det_count.p;
+    a.det_cap = cap;
+    c->exit_count = 0;
+    if (count > 0 && (rc = launch(c, a))) return rc;
+    const double fwd_ms = c->stats.kernel_ms;
+    unsigned long long n = 0;
+    HIPCHK(c, hipMemcpy(&n, c->d_det_count.p, 8, hipMemcpyDeviceToHost));
+    if ((int64_t)n > cap) return fail(c, LP_EDEVICE, "lp_adjoint: detection list overflow");
+    std::vector<DetRec> list(n);
+    if (n) HIPCHK(c, hipMemcpy(list.data(), c->d_det_list.p, n * sizeof(DetRec), hipMemcpyDeviceToHost));
+    if (signals) {
+        std::fill(signals, signals + nd, 0);
+        for (const DetRec& r : list) signals[r.det] += (int64_t)(r.w * 4294967296.0f);
+    }
+
+    // 2. replay the detected photons, accumulating the gradients
+    const size_t gbytes = (size_t)nd * nt * 2 * sizeof(int64_t);
+    const size_t jbytes = (jac && nvox > 0) ? (size_t)nd * nvox * sizeof(int64_t) : 0;
+    if ((rc = dev_alloc(c, c->d_grad, gbytes))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_grad.p, 0, gbytes, c->stream));
+    if (jbytes) {
+        if ((rc = dev_alloc(c, c->d_jac, jbytes))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->d_jac.p, 0, jbytes, c->stream));
+    }
+    if ((rc = dev_alloc(c, c->d_adj_misc, kMiscN * sizeof(int64_t)))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_adj_misc.p, 0, kMiscN * sizeof(int64_t), c->stream));
+    KArgs b;
+    fill_args(c, b);
+    b.sft = b.rtd = b.arz = b.phirz = b.region = b.vox = b.rad = b.det = nullptr;
+    b.misc = (unsigned long long*)c->d_adj_misc.p;
+    b.exit_rec = nullptr;
+    b.first = 0;
+    b.count = (int64_t)n;
+    b.adj_list = (const DetRec*)c->d_det_list.p;
+    b.grad = (unsigned long long*)c->d_grad.p;
+    b.jac = jbytes ? (unsigned long long*)c->d_jac.p : nullptr;
+    for (int k = 0; k < 3; ++k) {
+        b.jac_origin[k] = c->jac_origin[k];
+        b.jac_inv[k] = 1.0f / c->jac_size[k];
+        b.jac_n[k] = c->jac_n[k];
+    }
+    double rep_ms = 0;
+    if (n > 0) {
+        if ((rc = launch(c, b))) return rc;
+        rep_ms = c->stats.kernel_ms;
+    }
+    int64_t misc[kMiscN];
+    HIPCHK(c, hipMemcpy(misc, c->d_adj_misc.p, sizeof misc, hipMemcpyDeviceToHost));
+    if (grad) HIPCHK(c, hipMemcpy(grad, c->d_grad.p, gbytes, hipMemcpyDeviceToHost));
+    if (jbytes) HIPCHK(c, hipMemcpy(jac, c->d_jac.p, jbytes, hipMemcpyDeviceToHost));
+    if (npairs) *npairs = (int64_t)n;
+    c->stats.kernel_ms = fwd_ms;
+    c->stats.photons = count;
+    c->stats.replay_ms = rep_ms;
+    c->stats.replayed = (int64_t)n;
+    c->stats.replay_mismatch = misc[kMiscReplayMismatch];
+    return LP_OK;
+}
+
 int lp_get_stats(lp_context* c, lp_stats* out)
 {
     if (!c || !out) return LP_EINVAL;
     int64_t bytes = 0;
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
-                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris};
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list, &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_adj_misc};
     for (DevBuf* b : bufs) bytes += (int64_t)b->bytes;
     for (auto& b : c->d_tally) bytes += (int64_t)b.bytes;
     c->stats.device_bytes = bytes;

@@ -148,15 +148,38 @@ struct KArgs {
     float* trace;
     int32_t* trace_n;
     int32_t trace_max;
+    // detected photons (forward launch; null = off): one record per (photon,
+    // detector) pair whose exit radius lies in the detector's annulus
+    struct DetRec* det_list;
+    unsigned long long* det_count;
+    int64_t det_cap;
+    // adjoint replay (the ADJ kernel): photons come from adj_list[0, count);
+    // gradients by the path-length estimator, 2^32 fixed point
+    const struct DetRec* adj_list;
+    unsigned long long* grad;  // ndet * ntissue * 2: (d/dmua, d/dmus)
+    unsigned long long* jac;   // ndet * jac voxels (null = no voxel Jacobian)
+    float jac_origin[3], jac_inv[3];  // jac_inv = 1 / voxel size in float
+    int32_t jac_n[3];
     // work
     int64_t first, count;
     unsigned long long* counter;
 };
 
+// A detected photon: absolute photon index, detector, R exit weight.
+struct DetRec {
+    int64_t photon;
+    int32_t det;
+    float w;
+};
+static_assert(sizeof(DetRec) == 16, "detection record layout");
+
+constexpr int kAdjMaxTissues = 16;  // tissues the adjoint replay tracks per lane (LDS)
+
 enum MiscSlot {  // slots of the misc tally (kMiscN int64 words)
     kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
     kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscDistQueries,
-    kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds, kMiscUsed, kMiscN = 32
+    kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds, kMiscReplayMismatch, kMiscUsed,
+    kMiscN = 32
 };
 
 static_assert(kMiscUsed <= kMiscN, "misc tally too small for its counters");

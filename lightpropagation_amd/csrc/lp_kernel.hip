@@ -538,12 +538,96 @@ struct Photon {
     int64_t k;  // index within this run
 };
 
+// Adjoint: adds -(length in voxel) * W of the segment c0 -> c1 (length s,
+// emitter frame) to each voxel of the Jacobian grid it crosses --
+// Amanatides & Woo's traversal in grid units, clipped to the grid, one axis
+// advanced per crossing (x before y before z on ties); the same float
+// operations as the oracle's restatement, so the int64 sums agree bit for bit.
+__device__ __forceinline__ void jac_dda(const KArgs& a, unsigned long long* jd, f3 c0, f3 c1, float s, float W)
+{
+    const int32_t n0 = a.jac_n[0], n1 = a.jac_n[1], n2 = a.jac_n[2];
+    const float g0x = (c0.x - a.jac_origin[0]) * a.jac_inv[0];
+    const float g0y = (c0.y - a.jac_origin[1]) * a.jac_inv[1];
+    const float g0z = (c0.z - a.jac_origin[2]) * a.jac_inv[2];
+    const float dx = (c1.x - a.jac_origin[0]) * a.jac_inv[0] - g0x;
+    const float dy = (c1.y - a.jac_origin[1]) * a.jac_inv[1] - g0y;
+    const float dz = (c1.z - a.jac_origin[2]) * a.jac_inv[2] - g0z;
+    float t0 = 0.0f, t1 = 1.0f;
+    auto clip = [&](float g, float d, int32_t n) -> bool {
+        if (d == 0.0f) return g >= 0.0f && g < (float)n;
+        float ta = (0.0f - g) / d, tb = ((float)n - g) / d;
+        if (ta > tb) {
+            float x = ta;
+            ta = tb;
+            tb = x;
+        }
+        if (ta > t0) t0 = ta;
+        if (tb < t1) t1 = tb;
+        return true;
+    };
+    if (!clip(g0x, dx, n0) || !clip(g0y, dy, n1) || !clip(g0z, dz, n2)) return;
+    if (!(t0 < t1)) return;
+    const float inf = __builtin_inff();
+    auto setup = [&](float g, float d, int32_t n, int32_t& i, int32_t& st, float& tmax, float& tdel) {
+        int32_t k = (int32_t)__builtin_floorf(g + t0 * d);
+        if (k < 0) k = 0;
+        if (k > n - 1) k = n - 1;
+        i = k;
+        if (d > 0.0f) {
+            st = 1;
+            tmax = ((float)(k + 1) - g) / d;
+            tdel = 1.0f / d;
+        } else if (d < 0.0f) {
+            st = -1;
+            tmax = ((float)k - g) / d;
+            tdel = -1.0f / d;
+        } else {
+            st = 0;
+            tmax = inf;
+            tdel = inf;
+        }
+    };
+    int32_t ix, iy, iz, sx, sy, sz;
+    float mx, my, mz, ex, ey, ez;
+    setup(g0x, dx, n0, ix, sx, mx, ex);
+    setup(g0y, dy, n1, iy, sy, my, ey);
+    setup(g0z, dz, n2, iz, sz, mz, ez);
+    float t = t0;
+    for (int32_t it = 0; it < n0 + n1 + n2 + 3; ++it) {
+        const float tn = fminf(fminf(mx, my), fminf(mz, t1));
+        if (tn > t) {
+            const int64_t v = ((int64_t)iz * n1 + iy) * n0 + ix;
+            atomic_add_u64(&jd[v], fx32(-(((tn - t) * s) * W)));
+            t = tn;
+        }
+        if (t >= t1) break;
+        if (mx <= my && mx <= mz) {
+            ix += sx;
+            if (ix < 0 || ix >= n0) break;
+            mx += ex;
+        } else if (my <= mz) {
+            iy += sy;
+            if (iy < 0 || iy >= n1) break;
+            my += ey;
+        } else {
+            iz += sz;
+            if (iz < 0 || iz >= n2) break;
+            mz += ez;
+        }
+    }
+}
+
 // per-wavefront event counters in LDS
-enum WaveCount { wPhot, wInt, wBnd, wEsc, wCap, wSkip, wDq, wNan, wLst, wIter, wLRounds, wBRounds, wArzOut, wVoxOut, kWc };
+enum WaveCount {
+    wPhot, wInt, wBnd, wEsc, wCap, wSkip, wDq, wNan, wLst, wIter, wLRounds, wBRounds, wArzOut, wVoxOut, wMismatch,
+    kWc
+};
 
 // WAVES: minimum waves per SIMD the register allocator must allow (1 =
 // the compiler's choice); a tuning knob, results do not depend on it.
-template <int RNG, bool TRACE, int WAVES>
+// ADJ: the adjoint replay -- photons come from the detection list and every
+// move adds to the path-length gradients instead of the tallies.
+template <int RNG, bool TRACE, int WAVES, bool ADJ>
 __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
 {
     extern __shared__ float4 smem[];
@@ -551,6 +635,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     float4* lnodes = smem;                         // 2 * nlds
     float4* lmat = smem + 2 * a.nlds;              // ntissue
     int32_t* lran3 = reinterpret_cast<int32_t*>(lmat + kMaxTissues);
+    // adjoint: per-lane path length and interaction count per tissue, tissue-major
+    float* adjL = reinterpret_cast<float*>(reinterpret_cast<char*>(lran3) + (RNG == 1 ? sizeof(Ran3Lds) : 0));
+    uint32_t* adjN = reinterpret_cast<uint32_t*>(adjL + kAdjMaxTissues * kBlock);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -602,10 +689,26 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     bool search = false;     // the pending step needs a face search
     bool by_list = false;    // ... from the near-face lists (else from the BVH)
 
+    int32_t adj_det = 0;  // adjoint: the detector and R weight of the replayed photon
+    float adj_w = 0.0f;
+    // adjoint: one move of length s from p0 in the current tissue
+    auto adj_step = [&](f3 p0, float s, bool collision) {
+        adjL[ph.curr * kBlock + tid] += s;
+        if (collision) adjN[ph.curr * kBlock + tid] += 1u;
+        if (a.jac) {
+            const int64_t nvox = (int64_t)a.jac_n[0] * a.jac_n[1] * a.jac_n[2];
+            const f3 d0 = sub(p0, POI), d1 = sub(ph.p, POI);
+            jac_dda(a, a.jac + adj_det * nvox, mk(dot(eX, d0), dot(eY, d0), dot(eZ, d0)),
+                    mk(dot(eX, d1), dot(eY, d1), dot(eZ, d1)), s, adj_w);
+        }
+    };
+
     // interaction branch (.cl:1513-1537)
     auto interaction = [&](float s) {
         const float4 cm = lmat[ph.curr];
+        const f3 p_old = ph.p;
         ph.p = add(ph.p, mul(s, ph.v));
+        if constexpr (ADJ) adj_step(p_old, s, true);
         ph.ref = -1;
         if (cm.y > 0.0f && cm.z > 0.0f) {  // Absorb (.cl:841-847)
             ph.absorbed = ph.w * (cm.y / (cm.y + cm.z));
@@ -691,7 +794,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     auto boundary_event = [&](const Hit& h, float s, float stb) -> int {
         const float4 cm = lmat[ph.curr];
         int kind;
+        const f3 p_old = ph.p;
         ph.p = add(ph.p, mul(stb, ph.v));  // MoveStep
+        if constexpr (ADJ) adj_step(p_old, stb, false);
         ph.ref = -1;
         ph.safe = 0.0f;
         if (h.id < 0) {  // nothing ahead: the photon has left the mesh
@@ -824,6 +929,23 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         if (ph.p.x != ph.p.x || ph.p.y != ph.p.y || ph.p.z != ph.p.z || ph.v.x != ph.v.x || ph.v.y != ph.v.y ||
             ph.v.z != ph.v.z)
             wcount(wNan);
+        if constexpr (ADJ) {
+            // the replay must end as the forward photon did
+            if (ph.sft.x != adj_w) wcount(wMismatch);
+            const float W = adj_w;
+            for (int32_t t = 0; t < a.ntissue; ++t) {
+                const float L = adjL[t * kBlock + tid];
+                const uint32_t n = adjN[t * kBlock + tid];
+                if (L == 0.0f && n == 0u) continue;
+                const float mus = lmat[t].z;
+                const float gm = -(L * W);
+                const float gs = mus != 0.0f ? (((float)n / mus) - L) * W : gm;
+                unsigned long long* g = a.grad + ((int64_t)adj_det * a.ntissue + t) * 2;
+                atomic_add_u64(&g[0], fx32(gm));
+                atomic_add_u64(&g[1], fx32(gs));
+            }
+            return;
+        }
         if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
         reg_acc = 0;
         if (vox_cell >= 0) atomic_add_u64(&a.vox[vox_cell], vox_acc);
@@ -885,6 +1007,20 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 }
             }
         }
+        if (a.det_list && e3 != 0.0f) {  // detected photons, for the adjoint replay
+            for (int kd = 0; kd < a.ndet; ++kd) {
+                if (e7 >= a.det_rlo[kd] && e7 < a.det_rhi[kd]) {
+                    const unsigned long long slot = atomicAdd(a.det_count, 1ull);
+                    if (slot < (unsigned long long)a.det_cap) {
+                        DetRec r;
+                        r.photon = a.first + ph.k;
+                        r.det = kd;
+                        r.w = e3;
+                        a.det_list[slot] = r;
+                    }
+                }
+            }
+        }
         if ((lvis | ltst | ldvis) > 0x40000000u && a.misc) {  // rare: keep the 32-bit counts from wrapping
             atomic_add_u64(&a.misc[kMiscNodeVisits], lvis);
             atomic_add_u64(&a.misc[kMiscTriTests], ltst);
@@ -910,6 +1046,16 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     unsigned long long k = base + (unsigned long long)rank;
                     if (k < (unsigned long long)a.count) {
                         int64_t id = a.first + (int64_t)k;
+                        if constexpr (ADJ) {
+                            const DetRec r = a.adj_list[k];
+                            id = r.photon;
+                            adj_det = r.det;
+                            adj_w = r.w;
+                            for (int32_t t = 0; t < a.ntissue; ++t) {
+                                adjL[t * kBlock + tid] = 0.0f;
+                                adjN[t * kBlock + tid] = 0u;
+                            }
+                        }
                         rng.init(a.seed, (uint64_t)id);
                         // GetvStart (.cl:423-467): A == 1 -> vStart[0]; A > 1 by cumulative counts
                         int dsel = 0;
@@ -1142,6 +1288,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             atomic_add_u64(&a.misc[kMiscBvhRounds], wc[wBRounds]);
             atomic_add_u64(&a.misc[kMiscArzOut], wc[wArzOut]);
             atomic_add_u64(&a.misc[kMiscVoxOut], wc[wVoxOut]);
+            atomic_add_u64(&a.misc[kMiscReplayMismatch], wc[wMismatch]);
         }
     }
 }
@@ -1153,14 +1300,20 @@ int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errl
     const void* fn;
     if (a.rng_mode == 1) lds += sizeof(Ran3Lds);
     const int w = a.waves;
-#define LP_PICK(R, T, W) (const void*)transport_kernel<R, T, W>
-    if (a.rng_mode == 1)
-        fn = a.trace ? LP_PICK(1, true, 1) : LP_PICK(1, false, 1);
-    else if (a.trace)
-        fn = LP_PICK(0, true, 1);
-    else
-        fn = w == 4 ? LP_PICK(0, false, 4) : w == 5 ? LP_PICK(0, false, 5) : w == 6 ? LP_PICK(0, false, 6)
-                                                                        : LP_PICK(0, false, 1);
+#define LP_PICK(R, T, W, J) (const void*)transport_kernel<R, T, W, J>
+    if (a.adj_list) {
+        lds += (size_t)kAdjMaxTissues * kBlock * (sizeof(float) + sizeof(uint32_t));
+        fn = a.rng_mode == 1 ? LP_PICK(1, false, 1, true) : LP_PICK(0, false, 1, true);
+    } else if (a.rng_mode == 1) {
+        fn = a.trace ? LP_PICK(1, true, 1, false) : LP_PICK(1, false, 1, false);
+    } else if (a.trace) {
+        fn = LP_PICK(0, true, 1, false);
+    } else {
+        fn = w == 4   ? LP_PICK(0, false, 4, false)
+             : w == 5 ? LP_PICK(0, false, 5, false)
+             : w == 6 ? LP_PICK(0, false, 6, false)
+                      : LP_PICK(0, false, 1, false);
+    }
 #undef LP_PICK
     e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) {

@@ -572,11 +572,85 @@ static void record_exit(const orc_params *P, const orc_layout *L, orc_tallies *T
 }
 
 /* ------------------------------------------------------------------ */
+/* Adjoint bookkeeping of one photon                                     */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    float *L;       /* path length per tissue, accumulated in step order */
+    int32_t *n;     /* interactions per tissue */
+    float *seg;     /* 7 floats per move: start (3), end (3) in the emitter frame, length */
+    int64_t nseg, cap;
+} adj_track;
+
+static void adj_move(adj_track *A, int32_t tissue, of3 c0, of3 c1, float s, int collision)
+{
+    A->L[tissue] += s;
+    if (collision) A->n[tissue]++;
+    if (A->nseg == A->cap) {
+        A->cap = A->cap ? 2 * A->cap : 1024;
+        A->seg = (float *)realloc(A->seg, (size_t)A->cap * 7 * sizeof(float));
+    }
+    float *q = A->seg + 7 * A->nseg++;
+    q[0] = c0.x; q[1] = c0.y; q[2] = c0.z; q[3] = c1.x; q[4] = c1.y; q[5] = c1.z; q[6] = s;
+}
+
+/* Adds -(length in voxel) * W of the segment c0 -> c1 (length s) to each
+ * voxel it crosses: Amanatides & Woo's traversal in grid units, clipped to
+ * the grid; one axis advanced per crossing (x before y before z on ties). */
+static void dda_add(const orc_adjoint_out *A, int64_t *jac_d, const float *q, float W)
+{
+    const int32_t *n = A->jac_n;
+    float g0[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+        float inv = 1.0f / A->jac_size[a];
+        g0[a] = (q[a] - A->jac_origin[a]) * inv;
+        d[a] = (q[3 + a] - A->jac_origin[a]) * inv - g0[a];
+    }
+    float t0 = 0.0f, t1 = 1.0f;
+    for (int a = 0; a < 3; ++a) {
+        if (d[a] == 0.0f) {
+            if (!(g0[a] >= 0.0f && g0[a] < (float)n[a])) return;
+        } else {
+            float ta = (0.0f - g0[a]) / d[a], tb = ((float)n[a] - g0[a]) / d[a];
+            if (ta > tb) { float x = ta; ta = tb; tb = x; }
+            if (ta > t0) t0 = ta;
+            if (tb < t1) t1 = tb;
+        }
+    }
+    if (!(t0 < t1)) return;
+    int32_t i[3], st[3];
+    float tmax[3], tdel[3];
+    for (int a = 0; a < 3; ++a) {
+        int32_t k = (int32_t)floorf(g0[a] + t0 * d[a]);
+        if (k < 0) k = 0;
+        if (k > n[a] - 1) k = n[a] - 1;
+        i[a] = k;
+        if (d[a] > 0.0f) { st[a] = 1; tmax[a] = ((float)(k + 1) - g0[a]) / d[a]; tdel[a] = 1.0f / d[a]; }
+        else if (d[a] < 0.0f) { st[a] = -1; tmax[a] = ((float)k - g0[a]) / d[a]; tdel[a] = -1.0f / d[a]; }
+        else { st[a] = 0; tmax[a] = INFINITY; tdel[a] = INFINITY; }
+    }
+    const float s = q[6];
+    float t = t0;
+    for (int32_t it = 0; it < n[0] + n[1] + n[2] + 3; ++it) {
+        float tn = fminf(fminf(tmax[0], tmax[1]), fminf(tmax[2], t1));
+        if (tn > t) {
+            int64_t v = ((int64_t)i[2] * n[1] + i[1]) * n[0] + i[0];
+            add64(&jac_d[v], fx32(-(((tn - t) * s) * W)));
+            t = tn;
+        }
+        if (t >= t1) break;
+        int a = (tmax[0] <= tmax[1] && tmax[0] <= tmax[2]) ? 0 : (tmax[1] <= tmax[2] ? 1 : 2);
+        i[a] += st[a];
+        if (i[a] < 0 || i[a] >= n[a]) break;
+        tmax[a] += tdel[a];
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* One photon: kernel body .cl:1299-1598                                */
 /* ------------------------------------------------------------------ */
 static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc_tallies *T,
                        int64_t *reg_loc, int64_t *misc_loc, float *exit8, float *trace,
-                       int32_t trace_max, int32_t *trace_n)
+                       int32_t trace_max, int32_t *trace_n, adj_track *A)
 {
     const float *pb = P->poi_basis;
     const of3 POI = V(pb[0], pb[1], pb[2]);
@@ -634,7 +708,9 @@ static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc
         }
         if (stb <= stepsize) {
             /* boundary branch (.cl:1462-1511) */
+            of3 p_old = p;
             p = vadd(p, vmul(stb, v)); /* MoveStep */
+            if (A) adj_move(A, curr, calc_coords(p_old, POI, eX, eY, eZ), calc_coords(p, POI, eX, eY, eZ), stb, 0);
             refTri = -1;
             if (tri < 0) { /* no face ahead: the photon has left the mesh */
                 kind = ORC_EV_ESCAPE;
@@ -692,7 +768,9 @@ static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc
             misc_loc[ORC_MISC_BOUNDARY]++;
         } else {
             /* interaction branch (.cl:1513-1537) */
+            of3 p_old = p;
             p = vadd(p, vmul(stepsize, v));
+            if (A) adj_move(A, curr, calc_coords(p_old, POI, eX, eY, eZ), calc_coords(p, POI, eX, eY, eZ), stepsize, 1);
             refTri = -1;
             if (c_a > 0.0f && c_s > 0.0f) { /* Absorb (.cl:841-847) */
                 absorbed = w * (c_a / (c_a + c_s));
@@ -747,6 +825,10 @@ static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc
     if (exit8) memcpy(exit8, r8, sizeof r8);
     if (trace_n) *trace_n = ntr;
     record_exit(P, L, T, r8);
+    if (A) { /* R weight and exit radius for the detector test */
+        A->L[P->ntissue] = r8[3];
+        A->L[P->ntissue + 1] = r8[7];
+    }
 }
 
 int orc_run(const orc_params *P, int64_t first, int64_t count, orc_tallies *T, float *exit_rec,
@@ -772,7 +854,7 @@ int orc_run(const orc_params *P, int64_t first, int64_t count, orc_tallies *T, f
             int64_t id = first + i;
             run_photon(P, &L, id, T, reg_l, misc_l, exit_rec ? exit_rec + 8 * i : NULL,
                        trace ? trace + 12 * (int64_t)trace_max * i : NULL, trace_max,
-                       trace_n ? trace_n + i : NULL);
+                       trace_n ? trace_n + i : NULL, NULL);
         }
 #ifdef _OPENMP
 #pragma omp critical
@@ -790,3 +872,81 @@ int orc_run(const orc_params *P, int64_t first, int64_t count, orc_tallies *T, f
     free(reg);
     return 0;
 }
+
+int orc_adjoint(const orc_params *P, int64_t first, int64_t count, orc_tallies *T, orc_adjoint_out *out,
+                int nthreads)
+{
+    if (!P || !out || count < 0 || P->ntissue < 2 || P->nstart < 1) return 1;
+    orc_layout L;
+    orc_layout_compute(P, &L);
+    const int32_t nt = P->ntissue;
+    const int64_t nvox = (int64_t)out->jac_n[0] * out->jac_n[1] * out->jac_n[2];
+    int64_t misc[ORC_MISC_N] = {0}, pairs = 0;
+    int64_t *reg = (int64_t *)calloc((size_t)nt, sizeof(int64_t));
+    if (!reg) return 2;
+    orc_tallies none;
+    memset(&none, 0, sizeof none);
+    orc_tallies *TT = T ? T : &none;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads) reduction(+ : pairs)
+#endif
+    {
+        int64_t misc_l[ORC_MISC_N] = {0};
+        int64_t *reg_l = (int64_t *)calloc((size_t)nt, sizeof(int64_t));
+        adj_track A;
+        A.L = (float *)calloc((size_t)nt + 2, sizeof(float)); /* + R weight, exit radius */
+        A.n = (int32_t *)calloc((size_t)nt, sizeof(int32_t));
+        A.seg = NULL;
+        A.cap = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int64_t i = 0; i < count; ++i) {
+            memset(A.L, 0, ((size_t)nt + 2) * sizeof(float));
+            memset(A.n, 0, (size_t)nt * sizeof(int32_t));
+            A.nseg = 0;
+            run_photon(P, &L, first + i, TT, reg_l, misc_l, NULL, NULL, 0, NULL, &A);
+            const float W = A.L[nt], radius = A.L[nt + 1];
+            if (W == 0.0f) continue;
+            for (int32_t d = 0; d < P->ndet; ++d) {
+                if (!(radius >= P->det_rlo[d] && radius < P->det_rhi[d])) continue;
+                pairs++;
+                if (out->signals) add64(&out->signals[d], fx32(W));
+                if (out->grad)
+                    for (int32_t t = 0; t < nt; ++t) {
+                        if (A.L[t] == 0.0f && A.n[t] == 0) continue;
+                        float mus = P->nmuasg[4 * t + 2];
+                        float gm = -(A.L[t] * W);
+                        float gs = mus != 0.0f ? (((float)A.n[t] / mus) - A.L[t]) * W : gm;
+                        add64(&out->grad[((int64_t)d * nt + t) * 2], fx32(gm));
+                        add64(&out->grad[((int64_t)d * nt + t) * 2 + 1], fx32(gs));
+                    }
+                if (out->jac && nvox > 0)
+                    for (int64_t k = 0; k < A.nseg; ++k) dda_add(out, out->jac + d * nvox, A.seg + 7 * k, W);
+            }
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            for (int k = 0; k < ORC_MISC_N; ++k) misc[k] += misc_l[k];
+            for (int32_t k = 0; k < nt; ++k) reg[k] += reg_l[k];
+        }
+        free(reg_l);
+        free(A.L);
+        free(A.n);
+        free(A.seg);
+    }
+    out->ndetected += pairs;
+    if (T) {
+        misc[ORC_MISC_PHOTONS] += count;
+        for (int k = 0; k < ORC_MISC_N; ++k) T->misc[k] += misc[k];
+        if (T->region)
+            for (int32_t k = 0; k < nt; ++k) T->region[k] += reg[k];
+    }
+    free(reg);
+    return 0;
+}
+
+void orc_dda(const orc_adjoint_out *out, const float seg[7], float w, int64_t *jac) { dda_add(out, jac, seg, w); }

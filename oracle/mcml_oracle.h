@@ -158,6 +158,33 @@ int orc_run(const orc_params *p, int64_t first, int64_t count,
             orc_tallies *t, float *exit_rec, float *trace, int32_t trace_max,
             int32_t *trace_n, int nthreads);
 
+/* Adjoint (no reference counterpart): gradients of the annular detector
+ * signals S_d = sum of the R exit weights W of the photons whose exit radius
+ * lies in [det_rlo[d], det_rhi[d]), by the path-length (perturbation Monte
+ * Carlo) estimator on each detected path:
+ *   dW/dmua[t] = -L_t W,   dW/dmus[t] = (n_t / mus[t] - L_t) W   (mus[t] != 0)
+ * with L_t the path length and n_t the number of interactions in tissue t,
+ * and dW/dmua[v] = -L_v W for the path length L_v in voxel v of a grid in
+ * the emitter frame (3-D DDA, Amanatides & Woo 1987).  Sums are 2^32 fixed
+ * point, accumulated per (photon, detector) pair; photons [first,
+ * first+count) are also run through the ordinary tallies of orc_run when
+ * T is not NULL. */
+typedef struct orc_adjoint_out {
+    float jac_origin[3];
+    float jac_size[3];
+    int32_t jac_n[3];  /* any zero: no voxel Jacobian */
+    int64_t *signals;  /* ndet */
+    int64_t *grad;     /* ndet * ntissue * 2: (d/dmua, d/dmus) of tissue t */
+    int64_t *jac;      /* ndet * jac_n[0] * jac_n[1] * jac_n[2], x fastest */
+    int64_t ndetected; /* (photon, detector) pairs, accumulated */
+} orc_adjoint_out;
+
+int orc_adjoint(const orc_params *p, int64_t first, int64_t count, orc_tallies *t, orc_adjoint_out *out,
+                int nthreads);
+/* The voxel traversal alone, for the tests: adds -(length in voxel) * W of
+ * the segment seg = (start xyz, end xyz, length) to jac (one detector). */
+void orc_dda(const orc_adjoint_out *out, const float seg[7], float w, int64_t *jac);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,11 @@ class Tallies(C.Structure):
 _lib = None
 
 
+class AdjointOut(C.Structure):
+    _fields_ = [("jac_origin", C.c_float * 3), ("jac_size", C.c_float * 3), ("jac_n", C.c_int32 * 3),
+                ("signals", _lp), ("grad", _lp), ("jac", _lp), ("ndetected", C.c_int64)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -84,8 +89,24 @@ def lib():
         L.orc_run.restype = C.c_int
         L.orc_run.argtypes = [C.POINTER(Params), C.c_int64, C.c_int64, C.POINTER(Tallies), _fp, _fp,
                               C.c_int32, _ip, C.c_int]
+        L.orc_adjoint.restype = C.c_int
+        L.orc_adjoint.argtypes = [C.POINTER(Params), C.c_int64, C.c_int64, C.POINTER(Tallies),
+                                  C.POINTER(AdjointOut), C.c_int]
+        L.orc_dda.argtypes = [C.POINTER(AdjointOut), _fp, C.c_float, _lp]
         _lib = L
     return _lib
+
+
+def dda(seg, w, origin, size, n):
+    """orc_dda: -(length in voxel) * w per voxel (2^32 fixed point)."""
+    A = AdjointOut()
+    A.jac_origin[:] = list(origin)
+    A.jac_size[:] = list(size)
+    A.jac_n[:] = list(n)
+    jac = np.zeros(int(np.prod(n)), np.int64)
+    seg = np.ascontiguousarray(seg, np.float32)
+    lib().orc_dda(C.byref(A), fptr(seg), float(w), lptr(jac))
+    return jac
 
 
 def fptr(a):
@@ -226,6 +247,30 @@ class Run:
                            iptr(tn) if tn is not None else None, nthreads)
         assert rc == 0
         return ex, tr, tn
+
+    def adjoint(self, first, count, jac=None, nthreads=1, tallies=False):
+        """orc_adjoint: (signals (ndet,), grad (ndet, ntissue, 2), jac
+        (ndet, nvox) or None, pairs), 2^32 fixed point; jac = (origin,
+        size, n) of the voxel grid or None."""
+        nd = self.P.ndet
+        nt = self.P.ntissue
+        A = AdjointOut()
+        sig = np.zeros(max(nd, 1), np.int64)
+        grad = np.zeros(max(nd, 1) * nt * 2, np.int64)
+        J = None
+        if jac is not None:
+            o, sz, n = jac
+            A.jac_origin[:] = list(o)
+            A.jac_size[:] = list(sz)
+            A.jac_n[:] = list(n)
+            J = np.zeros(max(nd, 1) * int(np.prod(n)), np.int64)
+        A.signals, A.grad = lptr(sig), lptr(grad)
+        A.jac = lptr(J) if J is not None else None
+        rc = lib().orc_adjoint(C.byref(self.P), first, count, C.byref(self.T) if tallies else None, C.byref(A),
+                               nthreads)
+        assert rc == 0
+        return (sig[:nd], grad[:nd * nt * 2].reshape(nd, nt, 2),
+                J.reshape(max(nd, 1), -1)[:nd] if J is not None else None, A.ndetected)
 
     def tallies(self):
         out = {k: v.copy() for k, v in self.t.items()}

@@ -1,0 +1,121 @@
+"""The oracle's adjoint (path-length estimator of the detector-signal
+gradients) on the CPU, pinned by what the geometry and the estimator fix:
+
+* the voxel traversal gives each voxel the length of the segment inside it
+  (exact box/segment overlaps in double on random segments), and the lengths
+  add up to the segment;
+* per-voxel mua gradients summed over a grid that holds every path equal the
+  per-tissue mua gradients summed over tissues (both are -sum L W);
+* the per-tissue gradients equal the estimator evaluated on the oracle's own
+  step traces (path length per tissue from the positions, interactions per
+  tissue from the event kinds), so a dropped move, a move booked to the
+  tissue after a crossing instead of before, or a miscounted interaction
+  fails;
+* dS/dmus for a tissue with n interactions and length L is (n/mus - L) W:
+  checked on the same traces.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import scenes
+
+FX = 2.0 ** 32
+
+
+def _overlap(p0, p1, lo, hi):
+    """Length of segment p0 -> p1 inside the box [lo, hi) (double)."""
+    d = p1 - p0
+    t0, t1 = 0.0, 1.0
+    for a in range(3):
+        if d[a] == 0:
+            if not (lo[a] <= p0[a] < hi[a]):
+                return 0.0
+            continue
+        ta, tb = (lo[a] - p0[a]) / d[a], (hi[a] - p0[a]) / d[a]
+        ta, tb = min(ta, tb), max(ta, tb)
+        t0, t1 = max(t0, ta), min(t1, tb)
+    return max(0.0, t1 - t0) * float(np.linalg.norm(d))
+
+
+def test_dda_matches_exact_overlaps():
+    rng = np.random.default_rng(5)
+    origin, size, n = [-1.0, -0.5, 0.0], [0.25, 0.2, 0.1], [8, 6, 10]
+    for trial in range(300):
+        p0 = rng.uniform([-1.5, -1.0, -0.3], [1.5, 1.0, 1.3])
+        if trial % 3 == 0:  # axis-parallel and in-plane segments too
+            p1 = p0.copy()
+            p1[trial % 2] += rng.uniform(-1, 1)
+        else:
+            p1 = p0 + rng.normal(0, 0.5, 3)
+        p0f, p1f = p0.astype(np.float32), p1.astype(np.float32)
+        s = np.float32(np.linalg.norm(p1f.astype(np.float64) - p0f))
+        if s == 0:
+            continue
+        seg = np.concatenate([p0f, p1f, [s]]).astype(np.float32)
+        jac = O.dda(seg, 1.0, origin, size, n).reshape(n[2], n[1], n[0]) / -FX
+        want = np.zeros_like(jac)
+        for z in range(n[2]):
+            for y in range(n[1]):
+                for x in range(n[0]):
+                    lo = np.array(origin) + np.array([x, y, z]) * np.array(size)
+                    want[z, y, x] = _overlap(p0f.astype(np.float64), p1f.astype(np.float64), lo,
+                                             lo + np.array(size))
+        assert np.allclose(jac, want, rtol=1e-4, atol=2e-6 * max(float(s), 1.0)), trial
+
+
+def _c0_run(seed=3):
+    sc = scenes.slab(scenes.C0_LAYERS)
+    tall = {"det_rlo": [0.0, 0.3, 1.0], "det_rhi": [0.3, 1.0, 3.0]}
+    return sc, O.Run(sc, seed=seed, tallies=tall)
+
+
+def test_voxel_and_tissue_gradients_agree():
+    _, run = _c0_run()
+    # a grid holding every path: the 10 x 10 cm slab 3.15 cm deep, and the
+    # 0.002 cm of ambient above it where the pencil beam starts
+    grid = ([-5.0, -5.0, -0.01], [0.25, 0.25, 0.0205], [40, 40, 155])
+    sig, grad, jac, pairs = run.adjoint(0, 3000, jac=grid, nthreads=8)
+    assert pairs > 0
+    for d in range(3):
+        tissue = grad[d, :, 0].sum() / FX
+        voxels = jac[d].sum() / FX
+        assert abs(voxels - tissue) <= 1e-5 * abs(tissue) + 1e-6, (d, voxels, tissue)
+
+
+def test_gradients_equal_estimator_on_traces():
+    sc, run = _c0_run(seed=11)
+    N = 400
+    sig, grad, _, pairs = run.adjoint(0, N)
+    ex, tr, tn = run.run(0, N, exit_rec=True, trace_max=20000)
+    nt = len(sc["nmuasg"])
+    lo, hi = [0.0, 0.3, 1.0], [0.3, 1.0, 3.0]
+    want = np.zeros((3, nt, 2))
+    want_sig = np.zeros(3)
+    p0 = np.asarray(sc["pstart"], np.float64)
+    for i in range(N):
+        W, radius = ex[i, 3], ex[i, 7]
+        if W == 0:
+            continue
+        L = np.zeros(nt)
+        n = np.zeros(nt)
+        prev, tissue = p0, 0
+        for k in range(tn[i]):
+            rec = tr[i, k]
+            pos = rec[4:7].astype(np.float64)
+            L[tissue] += np.linalg.norm(pos - prev)  # the move happens in the tissue before the event
+            if rec[0] == 0:
+                n[tissue] += 1
+            prev, tissue = pos, int(rec[2])
+        for d in range(3):
+            if lo[d] <= radius < hi[d]:
+                want_sig[d] += W
+                for t in range(nt):
+                    mus = sc["nmuasg"][t][2]
+                    want[d, t, 0] += -L[t] * W
+                    want[d, t, 1] += ((n[t] / mus - L[t]) * W) if mus != 0 else -L[t] * W
+    assert np.allclose(sig / FX, want_sig, rtol=1e-6, atol=1e-6)
+    # the ambient start (tissue 0) is 0.002 cm long; the escape move of a
+    # photon is never in a detected path
+    assert np.allclose(grad / FX, want, rtol=2e-4, atol=1e-4)
+    assert np.any(grad[:, 1:, 1] != 0)

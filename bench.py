@@ -12,8 +12,10 @@ One step = one lp_run of --photons photons with inputs resident on the GPU.
 Prints one JSON line: the metric, the FP32-VALU roofline of the transport
 kernel (algorithmic work per photon step over the launch time measured with
 HIP events), its memory traffic per launch (from the committed rocprofv3
-FETCH_SIZE / WRITE_SIZE summary of this workload, scaled per photon), and
-the CPU baseline (the oracle on a bounded sample of the same workload).
+FETCH_SIZE / WRITE_SIZE summary of this workload, scaled per photon), the
+adjoint against central finite differences at equal gradient error (the
+metric's second half), and the CPU baseline (the oracle on a bounded sample
+of the same workload).
 """
 import argparse
 import json
@@ -91,6 +93,153 @@ def cpu_baseline(sc, seconds=20.0):
                       (done, steps, dt, steps / dt)}
 
 
+def adjoint_vs_fd(sc, photons=10_000_000, batches=4, rels=(1e-3, 2e-2), jac_n=64):
+    """Gradient of the 8 detector signals with respect to every tissue's mua
+    and mus (7 tissues x 2 = 14 parameters) and to mua in each voxel of a
+    jac_n^3 grid: the adjoint (lp_adjoint: forward launch + replay of the
+    detected photons) against central finite differences with common random
+    numbers (step rel x the value, two forward launches per parameter; the
+    voxel Jacobian would take 2 jac_n^3 of them), for each step in rels.
+    Both run the same `photons` in `batches` batches, whose spread gives each
+    estimate's standard error; the time a method needs for a given error
+    scales as time x SE^2, so the speedup at equal error is
+    (T_fd SE_fd^2) / (T_adj SE_adj^2), over the entries the adjoint resolves
+    (|mean| > 3 SE)."""
+    import lightpropagation_amd as lp
+    ctx = lp.Context(0)
+    ctx.set_scene(sc, seed=0x5EED0003)
+    lo, hi = detectors()
+    ctx.set_detectors(lo, hi)
+    ctx.set_tallies(1 << lp.TALLY["det"] | 1 << lp.TALLY["misc"])
+    # emitter frame: x, y across the abdomen, z into it; 64^3 voxels over 20.5 x 20.5 x 7.7 cm
+    ctx.set_jacobian_grid([-10.24, -10.24, 0.0], [0.32, 0.32, 0.12], [jac_n] * 3)
+    nm0 = np.asarray(sc["nmuasg"], np.float32)
+    params = [(t, c) for t in range(1, len(nm0)) for c in (1, 2)]
+    nd, P, B, n = len(lo), len(params), batches, photons // batches
+    adj = np.zeros((B, nd, P))
+    t_adj = fwd_ms = rep_ms = 0.0
+    pairs = 0
+    for b in range(B):
+        ctx.reset()
+        t0 = time.perf_counter()
+        r = ctx.adjoint(b * n, n, jac=True)
+        t_adj += time.perf_counter() - t0
+        st = ctx.stats()
+        fwd_ms += st.kernel_ms
+        rep_ms += st.replay_ms
+        pairs += r["pairs"]
+        if st.replay_mismatch:
+            raise RuntimeError("adjoint replay mismatch")
+        g = r["grad"] / lp.FIXED / n
+        for i, (t, c) in enumerate(params):
+            adj[b, :, i] = g[:, t, c - 1]
+    print("adjoint: %d x %d photons, %.1f s (forward %.0f ms, replay %.0f ms)" % (B, n, t_adj, fwd_ms, rep_ms),
+          file=sys.stderr, flush=True)
+    ma = adj.mean(0)
+    sa = adj.std(0, ddof=1) / math.sqrt(B)
+    res = np.abs(ma) > 3 * sa
+    kinds = np.array([c for _, c in params])[None, :].repeat(nd, 0)
+    out = {
+        "workload": "maternal-fetal mesh, %d photons in %d batches, 8 detectors x 14 parameters "
+                    "(7 tissues x mua, mus) + %d^3 voxel mua Jacobian; central differences with common random "
+                    "numbers" % (photons, B, jac_n),
+        "adjoint_s": t_adj, "adjoint_forward_ms": fwd_ms, "adjoint_replay_ms": rep_ms, "replayed_pairs": pairs,
+        "adjoint_resolved_entries": int(res.sum()), "entries": nd * P,
+        "fd_forward_passes": 2 * P, "fd_forward_passes_with_jacobian": 2 * P + 2 * jac_n ** 3, "fd": [],
+    }
+    for rel in rels:
+        fd = np.zeros((B, nd, P))
+        t_fd = 0.0
+        for i, (t, c) in enumerate(params):
+            h = float(nm0[t, c]) * rel
+            S = {}
+            for sgn in (-1, 1):
+                nm = nm0.copy()
+                nm[t, c] = np.float32(nm0[t, c] + sgn * h)
+                ctx.set_materials(nm, sc["fetal_id"])
+                S[sgn] = np.zeros((B, nd))
+                for b in range(B):
+                    ctx.reset()
+                    t0 = time.perf_counter()
+                    ctx.run(b * n, n)
+                    t_fd += time.perf_counter() - t0
+                    S[sgn][b] = ctx.tallies()["det"].reshape(-1, 2)[:, 0] / lp.FIXED / n
+            dh = float(np.float32(nm0[t, c] + h)) - float(np.float32(nm0[t, c] - h))
+            fd[:, :, i] = (S[1] - S[-1]) / dh
+        ctx.set_materials(nm0, sc["fetal_id"])
+        print("finite differences (step %g): %d forward launches of %d photons, %.1f s" % (rel, 2 * P * B, n, t_fd),
+              file=sys.stderr, flush=True)
+        mf = fd.mean(0)
+        sf = fd.std(0, ddof=1) / math.sqrt(B)
+        ok = res & (sf > 0)
+        speed = (t_fd * sf ** 2) / (t_adj * np.maximum(sa, 1e-300) ** 2)
+        agree = np.abs(ma - mf) <= 4 * np.sqrt(sa ** 2 + sf ** 2) + 0.05 * np.abs(ma)
+
+        def med(mask):
+            return float(np.median(speed[mask])) if mask.any() else None
+        t_fwd = t_fd / (2 * P)
+        out["fd"].append({
+            "step_rel": rel, "fd_s": t_fd, "forward_pass_s": t_fwd, "adjoint_over_forward": t_adj / t_fwd,
+            "fd_resolved_entries": int((np.abs(mf) > 3 * sf).sum()),
+            "agree_within_4se_5pct": int((agree & res).sum()), "compared_entries": int(ok.sum()),
+            "speedup_equal_error_median": med(ok), "speedup_equal_error_median_mua": med(ok & (kinds == 1)),
+            "speedup_equal_error_median_mus": med(ok & (kinds == 2)),
+            "speedup_equal_error_min": float(speed[ok].min()) if ok.any() else None,
+        })
+    ctx.close()
+    return out
+
+
+# Batched workload: 4 geometries (fetal head depth 1.5 / 2.5 / 3.5 / 4.5 cm;
+# icosphere frequency for about 5e4 / 2e5 / 2.8e5 / 3.9e5 faces; the 1.5 cm
+# geometry has a 0.3 cm fat layer and a 0.5 cm uterine wall so that the head
+# fits under them) x 2 wavelengths x 16 fetal SpO2 in [0.30, 0.75] (maternal
+# 0.97) = 128 instances, one batched launch of 32 instances per geometry.
+C2_GEOMS = [(1.5, 19, {"t_fat": 0.3, "t_uterus": 0.5}), (2.5, 38, {}), (3.5, 45, {}), (4.5, 53, {})]
+
+
+def batch_c2(photons=1_000_000, tune=True):
+    """Photons/s of the batched workload (lp_run_batch through its captured
+    HIP graph, 32 instances per launch; detector, region and counter
+    tallies), after the autotuner picked each geometry's launch options."""
+    import lightpropagation_amd as lp
+    from lightpropagation_amd import scenes
+    spo2 = np.linspace(0.30, 0.75, 16)
+    lo, hi = detectors()
+    out, total_ph, total_s = [], 0, 0.0
+    for depth, freq, kw in C2_GEOMS:
+        sc = scenes.maternal_fetal(fetal_depth=depth, freq_maternal=freq, freq_fetal=freq, **kw)
+        tab = np.stack([scenes.mf_optics(wl, 0.97, sf) for wl in (660.0, 940.0) for sf in spo2])
+        ctx = lp.Context(0)
+        ctx.set_scene(sc, seed=0x5EED0002)
+        ctx.set_detectors(lo, hi)
+        ctx.set_tallies(1 << lp.TALLY["det"] | 1 << lp.TALLY["region"] | 1 << lp.TALLY["misc"])
+        tuned = None
+        if tune:
+            r = ctx.autotune(sample=2_000_000, passes=2)
+            tuned = {"trials": r.trials, "ms_before": r.ms_before, "ms_after": r.ms_after, "waves": r.waves,
+                     "blocks_per_cu": r.blocks_per_cu, "free_iters": r.free_iters, "refill_min": r.refill_min,
+                     "list_min": r.list_min, "search_min": r.search_min}
+        ctx.run_batch(tab, 1 << 40, max(photons // 100, 1000))  # capture + warm-up
+        t0 = time.perf_counter()
+        b = ctx.run_batch(tab, 0, photons)
+        wall = time.perf_counter() - t0
+        kms = ctx.stats().kernel_ms
+        ctx.close()
+        n = len(tab) * photons
+        total_ph += n
+        total_s += wall
+        R = b["det"][:, :, 0].sum(1) / lp.FIXED / photons
+        out.append({"fetal_depth": depth, "faces": len(sc["tri"]), "instances": len(tab), "photons": n,
+                    "kernel_ms": kms, "wall_s": wall, "photons_per_s": n / wall, "autotune": tuned,
+                    "detected_R_660_range": [float(R[:16].min()), float(R[:16].max())]})
+        print("batch: depth %.1f, %d faces, %d x %d photons, %.0f ms" % (depth, len(sc["tri"]), len(tab), photons, kms),
+              file=sys.stderr, flush=True)
+    return {"workload": "4 geometries x 2 wavelengths x 16 fetal SpO2 = 128 instances, %d photons each, "
+                        "one graph-captured batched launch per geometry" % photons,
+            "photons_per_s": total_ph / total_s, "photons": total_ph, "seconds": total_s, "geometries": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +253,10 @@ def main():
     ap.add_argument("--lds-nodes", type=int, default=-1)
     ap.add_argument("--safety-cells", type=int, default=-1)
     ap.add_argument("--opt", action="append", default=[], help="tuning option key=value (lp_set_option)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint vs finite-difference measurement")
+    ap.add_argument("--adjoint-photons", type=int, default=10_000_000)
+    ap.add_argument("--c2", type=int, default=0, metavar="PHOTONS",
+                    help="also run the batched 128-instance workload with this many photons per instance")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run a plain copy of this many MiB (reference for profiler byte counters)")
     args = ap.parse_args()
@@ -198,6 +351,10 @@ def main():
         out["copy_calibration"] = {"bytes": args.calibrate_copy << 20, "ms": ms,
                                    "GB_per_s": 2 * (args.calibrate_copy << 20) / ms / 1e6}
     ctx.close()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_vs_fd(sc, photons=args.adjoint_photons)
+    if args.c2:
+        out["batch_c2"] = batch_c2(args.c2)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
     print(json.dumps(out))

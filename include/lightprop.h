@@ -213,7 +213,42 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
                                    of a wavefront's live lanes waits for one (0..1) */
 #define LP_OPT_REFILL_MIN 12    /* start new photons once this many lanes of a
                                    wavefront have finished (1..64)                 */
+#define LP_OPT_GRAPH 13         /* lp_run_batch through a captured HIP graph (1, default)
+                                   or eager launches (0)                           */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
+
+/* ---- batched launch (replaces ExecKernel's loop of 6 400-photon launches,
+ * CalculatePropagation.cpp:1446-1488, when many optical-property sets share
+ * one mesh) ----
+ * Runs photons [first, first + count) of each of ninst instances of the
+ * scene in one persistent launch.  Instance i has the tissue optics
+ * nmuasg[i][ntissue][4] = (n, mua, mus, g) (ntissue and the fetal id as set
+ * by lp_set_materials; Rsp per instance) and draws photon j from the same
+ * (seed, j) stream as lp_run, so instance i's detector and region tallies
+ * equal those of lp_set_materials(instance i) + lp_run bit for bit.
+ * Outputs, 2^32 fixed point, either may be NULL: det[ninst][ndet][2] (R, fetal
+ * R, as LP_TALLY_DET), region[ninst][ntissue] (as LP_TALLY_REGION).  The
+ * context's own tallies are untouched; event counters add to LP_TALLY_MISC.
+ * The launch (tally clears + transport) is captured once as a HIP graph and
+ * replayed while the scene, options and (ninst, first, count) stay the same;
+ * new optics only refill the device table.  ninst * (ntissue + 1) <= 1152. */
+int lp_run_batch(lp_context *ctx, const float *nmuasg, int32_t ninst, int64_t first, int64_t count, int64_t *det,
+                 int64_t *region);
+
+/* Launch-option autotuner: coordinate search over waves per SIMD,
+ * work-groups per CU, free iterations, refill threshold and the two search
+ * batch thresholds, timing tally-free launches of `sample` photons (a
+ * setting must win by 1 %), for up to `passes` passes; leaves the fastest
+ * setting on the context.  The options change how faces are found, never
+ * which, so results do not depend on the outcome. */
+typedef struct lp_tune_result {
+    int64_t sample_photons;
+    int32_t trials;
+    double ms_before, ms_after; /* sample launch time with the initial and the chosen options */
+    int32_t waves, blocks_per_cu, free_iters, refill_min;
+    double list_min, search_min;
+} lp_tune_result;
+int lp_autotune(lp_context *ctx, int64_t sample, int32_t passes, lp_tune_result *out);
 
 #ifdef __cplusplus
 }

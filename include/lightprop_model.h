@@ -41,9 +41,9 @@ void lpm_mf_default(lpm_mf_params *p);
 int64_t lpm_mf_num_triangles(const lpm_mf_params *p);
 int lpm_mf_build(const lpm_mf_params *p, float *tri, float *nrm, int32_t *med, int64_t ntri);
 
-/* Optical properties (n, mua, mus, g) of the 8 maternal-fetal tissues at a
- * wavelength (660 or 940 nm, or anything in 600-1000 nm by interpolation)
- * for maternal and fetal arterial oxygen saturations. */
+/* Optical properties (n, mua, mus, g) of the 8 maternal-fetal tissues at
+ * 660 or 940 nm (the two rows of the haemoglobin table; other wavelengths
+ * return 3) for maternal and fetal arterial oxygen saturations. */
 int lpm_mf_optics(double wavelength_nm, double spo2_maternal, double spo2_fetal, float *nmuasg, int32_t ntissue);
 
 /* Planar layered slab in the reference convention: x, y in [-hx, hx] x

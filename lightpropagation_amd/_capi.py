@@ -19,7 +19,7 @@ MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped", "arz_ou
         "bvh_rounds"]
 OPTIONS = {"grid_blocks": 0, "lds_nodes": 1, "safety_cells": 2, "dist_cap": 3, "dist_cap_steps": 4,
            "blocks_per_cu": 5, "free_iters": 6, "waves_per_simd": 7, "near_r0": 8, "near_cell": 9, "search_min": 10,
-           "list_min": 11, "refill_min": 12}
+           "list_min": 11, "refill_min": 12, "graph": 13}
 FIXED = 2.0 ** 32
 FIXED_PHI = 2.0 ** 24
 
@@ -32,6 +32,13 @@ class Layout(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("dimx", "dimy", "dimz", "dimR", "dimAlpha", "sft_nx", "sft_ny",
                                          "arz_nr", "arz_nz")] + \
                [("dist_min", C.c_float), ("rsp", C.c_float), ("n", C.c_int64 * 9)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("sample_photons", C.c_int64), ("trials", C.c_int32), ("ms_before", C.c_double),
+                ("ms_after", C.c_double), ("waves", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("free_iters", C.c_int32), ("refill_min", C.c_int32), ("list_min", C.c_double),
+                ("search_min", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -68,6 +75,8 @@ _SIGS = {
     "lp_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "lp_set_jacobian_grid": (C.c_int, [C.c_void_p, _fp, _fp, _ip]),
     "lp_adjoint": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _lp, _lp, _lp, _lp]),
+    "lp_run_batch": (C.c_int, [C.c_void_p, _fp, C.c_int32, C.c_int64, C.c_int64, _lp, _lp]),
+    "lp_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(TuneResult)]),
     "lp_debug_math": (C.c_int, [C.c_void_p, C.c_int32, _fp, _fp, C.c_int64]),
     "lp_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "lp_set_option": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
@@ -251,6 +260,27 @@ class Context:
                                    grad.ctypes.data_as(_lp), J.ctypes.data_as(_lp) if J is not None else None,
                                    C.byref(pairs)), "lp_adjoint")
         return dict(signals=sig, grad=grad, jac=J, pairs=pairs.value)
+
+    def run_batch(self, nmuasg, first, count):
+        """lp_run_batch: nmuasg (ninst, ntissue, 4) -> dict(det (ninst, ndet,
+        2), region (ninst, ntissue)) as int64 fixed point (2^32 per unit)."""
+        m = np.ascontiguousarray(nmuasg, np.float32)
+        if m.ndim != 3 or m.shape[2] != 4:
+            raise ValueError("nmuasg must be (ninst, ntissue, 4)")
+        ni, nt = m.shape[0], m.shape[1]
+        nd = self._keep.get("ndet", 0)
+        det = np.zeros((ni, max(nd, 0), 2), np.int64)
+        reg = np.zeros((ni, nt), np.int64)
+        self._chk(lib().lp_run_batch(self._c, fp(m), ni, int(first), int(count),
+                                     det.ctypes.data_as(_lp) if nd else None, reg.ctypes.data_as(_lp)),
+                  "lp_run_batch")
+        return dict(det=det, region=reg)
+
+    def autotune(self, sample=2_000_000, passes=2):
+        """lp_autotune: returns the TuneResult (options left on the context)."""
+        r = TuneResult()
+        self._chk(lib().lp_autotune(self._c, int(sample), int(passes), C.byref(r)), "lp_autotune")
+        return r
 
     def stats(self):
         s = Stats()

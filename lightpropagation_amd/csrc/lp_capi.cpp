@@ -82,6 +82,16 @@ struct lp_context {
     DevBuf d_det_list, d_det_count, d_grad, d_jac, d_adj_misc;
     int64_t exit_count = 0;
     lp_stats stats{};
+
+    // batched launches: instance material table, per-instance tallies, and
+    // the captured graph (zero the tallies and the work counter, transport)
+    // with the kernel arguments and grid it was captured for
+    DevBuf d_inst_mat, d_bdet, d_bregion;
+    int32_t use_graph = 1;
+    hipGraph_t bgraph = nullptr;
+    hipGraphExec_t bexec = nullptr;
+    KArgs bargs{};
+    int bgrid = 0;
 };
 
 namespace {
@@ -355,17 +365,24 @@ void fill_args(lp_context* c, KArgs& a)
         a.det_rhi[k] = c->det_rhi[k];
     }
     a.counter = (unsigned long long*)c->d_counter.p;
+    a.ninst = 1;
+    a.lmat_n = kMaxTissues + 1;
 }
 
-int launch(lp_context* c, KArgs& a)
+int grid_for(const lp_context* c, int64_t count)
 {
     int grid = c->grid_override;
     if (grid <= 0) {
         // enough work-groups to fill every CU at the occupancy the kernel allows
         grid = std::max(1, c->num_cus) * c->blocks_per_cu;
     }
-    int64_t max_grid = (a.count + kBlock - 1) / kBlock;
-    grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, std::max<int64_t>(max_grid, 1)));
+    int64_t max_grid = (count + kBlock - 1) / kBlock;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(grid, std::max<int64_t>(max_grid, 1)));
+}
+
+int launch(lp_context* c, KArgs& a)
+{
+    const int grid = grid_for(c, a.count);
     HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, 64, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     char err[256] = {0};
@@ -415,8 +432,12 @@ void lp_destroy(lp_context* c)
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->bexec) hipGraphExecDestroy(c->bexec);
+    if (c->bgraph) hipGraphDestroy(c->bgraph);
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
-                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list, &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_adj_misc};
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list,
+                      &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_adj_misc, &c->d_inst_mat, &c->d_bdet,
+                      &c->d_bregion};
     for (DevBuf* b : bufs) dev_free(*b);
     for (auto& b : c->d_tally) dev_free(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -593,6 +614,10 @@ int lp_set_option(lp_context* c, int32_t key, double v)
         if ((float)v != c->near_cell) c->mesh_dirty = true;
         c->near_cell = (float)v;
         return LP_OK;
+    case LP_OPT_GRAPH:
+        if (!(v == 0 || v == 1)) return fail(c, LP_EINVAL, "graph option must be 0 or 1");
+        c->use_graph = (int32_t)v;
+        return LP_OK;
     case LP_OPT_SEARCH_MIN:
         if (!(v >= 0 && v <= 1)) return fail(c, LP_EINVAL, "search fraction must be in [0, 1]");
         c->search_min = (float)v;
@@ -648,6 +673,207 @@ int lp_run(lp_context* c, int64_t first, int64_t count)
         c->exit_count = 0;
     }
     return launch(c, a);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Captures [zero the instance tallies, zero the work counter, transport
+// launch] on the context's stream as a graph, unless the graph held was
+// captured for the same kernel arguments and grid.  Memory is cleared by a
+// kernel node: memset nodes are not used in captured graphs.
+int batch_graph(lp_context* c, const KArgs& a, int grid, int64_t ndet_words, int64_t nreg_words)
+{
+    if (c->bexec && c->bgrid == grid && std::memcmp(&c->bargs, &a, sizeof a) == 0) return LP_OK;
+    if (c->bexec) hipGraphExecDestroy(c->bexec);
+    if (c->bgraph) hipGraphDestroy(c->bgraph);
+    c->bexec = nullptr;
+    c->bgraph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    char err[256] = {0};
+    int bad = 0;
+    if (a.det) bad |= launch_zero_u64(a.det, ndet_words, c->stream);
+    bad |= launch_zero_u64(a.region, nreg_words, c->stream);
+    bad |= launch_zero_u64(a.counter, 8, c->stream);
+    if (!bad) bad = launch_transport(a, grid, c->stream, err, sizeof err);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (bad || e != hipSuccess || !g) {
+        if (g) hipGraphDestroy(g);
+        return fail(c, LP_EDEVICE, "lp_run_batch: graph capture failed (%s)", err[0] ? err : hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return fail(c, LP_EDEVICE, "lp_run_batch: graph instantiation: %s", hipGetErrorString(e));
+    }
+    c->bgraph = g;
+    c->bexec = x;
+    c->bargs = a;
+    c->bgrid = grid;
+    return LP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lp_run_batch(lp_context* c, const float* nmuasg, int32_t ninst, int64_t first, int64_t count, int64_t* det,
+                 int64_t* region)
+{
+    if (!c) return LP_EINVAL;
+    if (!nmuasg || ninst < 1 || first < 0 || count < 0) return fail(c, LP_EINVAL, "lp_run_batch: bad arguments");
+    int rc = prepare(c);
+    if (rc) return rc;
+    const int32_t nt = c->ntissue, nd = (int32_t)c->det_rlo.size();
+    if ((int64_t)ninst * (nt + 1) > kMaxInstRows)
+        return fail(c, LP_EINVAL, "lp_run_batch: %d instances of %d tissues exceed the %d-row material table", ninst,
+                    nt, kMaxInstRows);
+    if (count > 0 && (int64_t)ninst > INT64_MAX / 2 / count) return fail(c, LP_EINVAL, "lp_run_batch: too many photons");
+    const int64_t ndw = (int64_t)ninst * nd * 2, nrw = (int64_t)ninst * nt;
+    std::vector<float> tab((size_t)ninst * (nt + 1) * 4, 0.0f);
+    for (int32_t i = 0; i < ninst; ++i) {
+        const float* m = nmuasg + (size_t)i * nt * 4;
+        for (int32_t t = 0; t < nt; ++t)
+            if (!(m[4 * t] > 0) || !(m[4 * t + 1] >= 0) || !(m[4 * t + 2] >= 0) || !(m[4 * t + 3] >= -1 && m[4 * t + 3] <= 1))
+                return fail(c, LP_EINVAL, "lp_run_batch: instance %d tissue %d: need n > 0, mua, mus >= 0, |g| <= 1", i, t);
+        std::memcpy(&tab[(size_t)i * (nt + 1) * 4], m, (size_t)nt * 4 * sizeof(float));
+        tab[((size_t)i * (nt + 1) + nt) * 4] = rspecular(m, nt);
+    }
+    if ((rc = dev_alloc(c, c->d_inst_mat, tab.size() * sizeof(float)))) return rc;
+    if ((rc = dev_alloc(c, c->d_bdet, (size_t)std::max<int64_t>(ndw, 1) * sizeof(int64_t)))) return rc;
+    if ((rc = dev_alloc(c, c->d_bregion, (size_t)nrw * sizeof(int64_t)))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_inst_mat.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    KArgs a;
+    fill_args(c, a);
+    a.sft = a.rtd = a.arz = a.phirz = a.vox = a.rad = nullptr;
+    a.det = nd > 0 ? (unsigned long long*)c->d_bdet.p : nullptr;
+    a.region = (unsigned long long*)c->d_bregion.p;
+    a.inst_mat = (const float*)c->d_inst_mat.p;
+    a.ninst = ninst;
+    a.lmat_n = std::max<int32_t>(kMaxTissues + 1, ninst * (nt + 1));
+    a.inst_count = std::max<int64_t>(count, 1);
+    a.first = first;
+    a.count = (int64_t)ninst * count;
+    const int grid = grid_for(c, a.count);
+    if (a.count == 0) {
+        if (det) std::memset(det, 0, (size_t)ndw * sizeof(int64_t));
+        if (region) std::memset(region, 0, (size_t)nrw * sizeof(int64_t));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return LP_OK;
+    }
+    if (c->use_graph) {
+        if ((rc = batch_graph(c, a, grid, ndw, nrw))) return rc;
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        HIPCHK(c, hipGraphLaunch(c->bexec, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    } else {
+        if (a.det) HIPCHK(c, hipMemsetAsync(a.det, 0, (size_t)ndw * sizeof(int64_t), c->stream));
+        HIPCHK(c, hipMemsetAsync(a.region, 0, (size_t)nrw * sizeof(int64_t), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, 64, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        char err[256] = {0};
+        if (launch_transport(a, grid, c->stream, err, sizeof err)) return fail(c, LP_EDEVICE, "%s", err);
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    }
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (det && ndw) HIPCHK(c, hipMemcpy(det, c->d_bdet.p, (size_t)ndw * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (region) HIPCHK(c, hipMemcpy(region, c->d_bregion.p, (size_t)nrw * sizeof(int64_t), hipMemcpyDeviceToHost));
+    c->stats.kernel_ms = ms;
+    c->stats.photons = a.count;
+    c->stats.grid_blocks = grid;
+    return LP_OK;
+}
+
+int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* out)
+{
+    if (!c) return LP_EINVAL;
+    if (sample < 1 || passes < 1 || passes > 8) return fail(c, LP_EINVAL, "lp_autotune: sample >= 1, 1..8 passes");
+    int rc = prepare(c);
+    if (rc) return rc;
+    if ((rc = dev_alloc(c, c->d_adj_misc, kMiscN * sizeof(int64_t)))) return rc;
+    // one tally-free sample launch (event counters into a scratch row)
+    auto trial = [&](double* ms) -> int {
+        KArgs a;
+        fill_args(c, a);
+        a.sft = a.rtd = a.arz = a.phirz = a.region = a.vox = a.rad = a.det = nullptr;
+        a.misc = (unsigned long long*)c->d_adj_misc.p;
+        a.first = 0;
+        a.count = sample;
+        int r = launch(c, a);
+        *ms = c->stats.kernel_ms;
+        return r;
+    };
+    struct Knob {
+        const char* name;
+        std::vector<double> values;
+        double* cur;
+    };
+    double w = c->waves, bpc = c->blocks_per_cu, fi = c->free_iters, rm = c->refill_min, lm = c->list_min,
+           sm = c->search_min;
+    std::vector<Knob> knobs = {{"waves", {1, 4, 5, 6}, &w},
+                               {"blocks_per_cu", {2, 3, 4, 5, 6, 8}, &bpc},
+                               {"free_iters", {16, 32, 64}, &fi},
+                               {"refill_min", {8, 16, 24, 32}, &rm},
+                               {"list_min", {0.125, 0.25, 0.5}, &lm},
+                               {"search_min", {0.5, 0.75, 1.0}, &sm}};
+    auto apply = [&]() {
+        c->waves = (int32_t)w;
+        c->blocks_per_cu = (int32_t)bpc;
+        c->free_iters = (int32_t)fi;
+        c->refill_min = (int32_t)rm;
+        c->list_min = (float)lm;
+        c->search_min = (float)sm;
+    };
+    double ms0 = 0, best = 0, tmp = 0;
+    if ((rc = trial(&tmp))) return rc;  // warm-up (first launch of the variant)
+    if ((rc = trial(&ms0))) return rc;
+    best = ms0;
+    int trials = 2;
+    for (int pass = 0; pass < passes; ++pass) {
+        bool changed = false;
+        for (Knob& k : knobs) {
+            const double keep = *k.cur;
+            double kbest = keep;
+            for (double v : k.values) {
+                if (v == keep) continue;
+                *k.cur = v;
+                apply();
+                double ms = 0;
+                if ((rc = trial(&ms))) {
+                    *k.cur = keep;
+                    apply();
+                    return rc;
+                }
+                ++trials;
+                if (ms < best * 0.99) {  // 1 % better than the best so far
+                    best = ms;
+                    kbest = v;
+                }
+            }
+            *k.cur = kbest;
+            apply();
+            changed |= kbest != keep;
+        }
+        if (!changed) break;
+    }
+    if (out) {
+        out->sample_photons = sample;
+        out->trials = trials;
+        out->ms_before = ms0;
+        out->ms_after = best;
+        out->waves = c->waves;
+        out->blocks_per_cu = c->blocks_per_cu;
+        out->free_iters = c->free_iters;
+        out->refill_min = c->refill_min;
+        out->list_min = c->list_min;
+        out->search_min = c->search_min;
+    }
+    return LP_OK;
 }
 
 int lp_get_tally(lp_context* c, int32_t which, int64_t* out, int64_t n)
@@ -823,7 +1049,9 @@ int lp_get_stats(lp_context* c, lp_stats* out)
     if (!c || !out) return LP_EINVAL;
     int64_t bytes = 0;
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
-                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list, &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_adj_misc};
+                      &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list,
+                      &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_adj_misc, &c->d_inst_mat, &c->d_bdet,
+                      &c->d_bregion};
     for (DevBuf* b : bufs) bytes += (int64_t)b->bytes;
     for (auto& b : c->d_tally) bytes += (int64_t)b.bytes;
     c->stats.device_bytes = bytes;

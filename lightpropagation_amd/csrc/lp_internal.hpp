@@ -160,6 +160,16 @@ struct KArgs {
     unsigned long long* jac;   // ndet * jac voxels (null = no voxel Jacobian)
     float jac_origin[3], jac_inv[3];  // jac_inv = 1 / voxel size in float
     int32_t jac_n[3];
+    // batched launch (inst_mat null = one instance with nmuasg and rsp):
+    // ninst instances differing in their tissue optics, instance i's
+    // materials at inst_mat[i * (ntissue + 1) + t] and its (rsp, 0, 0, 0) at
+    // [i * (ntissue + 1) + ntissue]; photon slot k of [0, count) is index
+    // first + k % inst_count of instance k / inst_count; det and region
+    // tallies are per instance
+    const float* inst_mat;  // 4 floats per row, 16-byte aligned
+    int32_t ninst;
+    int32_t lmat_n;  // float4 rows of the material table in LDS (>= ninst * (ntissue + 1))
+    int64_t inst_count;
     // work
     int64_t first, count;
     unsigned long long* counter;
@@ -188,5 +198,8 @@ static_assert(kMiscUsed <= kMiscN, "misc tally too small for its counters");
 int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen);
 int launch_debug_math(int op, const float* in, float* out, int64_t n, void* stream);
 int launch_debug_copy(const void* src, void* dst, int64_t bytes, void* stream);
+int launch_zero_u64(void* p, int64_t n, void* stream);
+// rows of the per-instance material table a batched launch may stage in LDS
+constexpr int kMaxInstRows = 1152;
 
 }  // namespace lp

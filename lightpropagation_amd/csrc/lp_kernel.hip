@@ -534,6 +534,7 @@ struct Photon {
     float safe, pc;
     int32_t ref, curr;
     int32_t fetal, roulette;
+    int32_t inst;  // instance of a batched launch (0 otherwise)
     int64_t steps;
     int64_t k;  // index within this run
 };
@@ -633,8 +634,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     extern __shared__ float4 smem[];
     __shared__ unsigned long long wcnt[kBlock / 64][kWc];
     float4* lnodes = smem;                         // 2 * nlds
-    float4* lmat = smem + 2 * a.nlds;              // ntissue
-    int32_t* lran3 = reinterpret_cast<int32_t*>(lmat + kMaxTissues);
+    float4* lmat = smem + 2 * a.nlds;              // lmat_n: per instance, ntissue materials + (rsp, 0, 0, 0)
+    int32_t* lran3 = reinterpret_cast<int32_t*>(lmat + a.lmat_n);
     // adjoint: per-lane path length and interaction count per tissue, tissue-major
     float* adjL = reinterpret_cast<float*>(reinterpret_cast<char*>(lran3) + (RNG == 1 ? sizeof(Ran3Lds) : 0));
     uint32_t* adjN = reinterpret_cast<uint32_t*>(adjL + kAdjMaxTissues * kBlock);
@@ -645,10 +646,17 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         const float4* gn = reinterpret_cast<const float4*>(a.nodes);
         for (int i = tid; i < 2 * a.nlds; i += kBlock) lnodes[i] = gn[i];
         for (int i = tid; i < (kBlock / 64) * kWc; i += kBlock) (&wcnt[0][0])[i] = 0;
-        for (int i = tid; i < a.ntissue; i += kBlock)
-            lmat[i] = make_float4(a.nmuasg[4 * i], a.nmuasg[4 * i + 1], a.nmuasg[4 * i + 2], a.nmuasg[4 * i + 3]);
+        if (a.inst_mat) {
+            const float4* im = reinterpret_cast<const float4*>(a.inst_mat);
+            for (int i = tid; i < a.ninst * (a.ntissue + 1); i += kBlock) lmat[i] = im[i];
+        } else {
+            for (int i = tid; i < a.ntissue; i += kBlock)
+                lmat[i] = make_float4(a.nmuasg[4 * i], a.nmuasg[4 * i + 1], a.nmuasg[4 * i + 2], a.nmuasg[4 * i + 3]);
+            if (tid == 0) lmat[a.ntissue] = make_float4(a.rsp, 0.0f, 0.0f, 0.0f);
+        }
     }
     __syncthreads();
+    const int32_t ntp1 = a.ntissue + 1;  // material rows per instance
 
     const f3 POI = mk(a.poi[0], a.poi[1], a.poi[2]);
     const f3 eX = mk(a.ex[0], a.ex[1], a.ex[2]);
@@ -705,7 +713,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
 
     // interaction branch (.cl:1513-1537)
     auto interaction = [&](float s) {
-        const float4 cm = lmat[ph.curr];
+        const float4 cm = lmat[ph.inst * ntp1 + ph.curr];
         const f3 p_old = ph.p;
         ph.p = add(ph.p, mul(s, ph.v));
         if constexpr (ADJ) adj_step(p_old, s, true);
@@ -792,7 +800,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
 
     // boundary branch (.cl:1462-1511) at face h.id, distance stb
     auto boundary_event = [&](const Hit& h, float s, float stb) -> int {
-        const float4 cm = lmat[ph.curr];
+        const float4 cm = lmat[ph.inst * ntp1 + ph.curr];
         int kind;
         const f3 p_old = ph.p;
         ph.p = add(ph.p, mul(stb, ph.v));  // MoveStep
@@ -808,7 +816,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             int32_t in_id = h.med & 0xffff, out_id = (h.med >> 16) & 0xffff;
             ph.nv = normalize(mk(a.nrm[3 * h.id], a.nrm[3 * h.id + 1], a.nrm[3 * h.id + 2]));
             int32_t nx_id = (ph.curr == out_id) ? in_id : out_id;
-            const float4 nm = lmat[nx_id];
+            const float4 nm = lmat[ph.inst * ntp1 + nx_id];
             if (__builtin_fabsf(__builtin_sqrtf(ph.p.x * ph.p.x + ph.p.y * ph.p.y)) < 0.01f)
                 ph.nv = mul(sign_cl(dot(ph.nv, eZr)), eZ);
             // TransmitOrReflect (.cl:921-1018)
@@ -868,7 +876,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 // transmitted past distMin, else reflected (and fetal-tagged);
                 // written as value selects so that sft stays in registers
                 float dist = __builtin_fabsf(dot(eZ, sub(ph.p, POI)));
-                float wr = ph.w * (1 - a.rsp);
+                float wr = ph.w * (1 - lmat[ph.inst * ntp1 + a.ntissue].x);  // (1 - Rsp)
                 bool trans = dist > a.dist_min;
                 ph.sft.z = trans ? wr : ph.sft.z;
                 ph.sft.x = trans ? ph.sft.x : wr;
@@ -876,7 +884,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 ph.w = 0.0f;
             }
             if (new_id != ph.curr) {
-                if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
+                if (a.region) atomic_add_u64(&a.region[ph.inst * a.ntissue + ph.curr], reg_acc);
                 reg_acc = 0;
             }
             ph.curr = new_id;
@@ -946,7 +954,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             }
             return;
         }
-        if (a.region) atomic_add_u64(&a.region[ph.curr], reg_acc);
+        if (a.region) atomic_add_u64(&a.region[ph.inst * a.ntissue + ph.curr], reg_acc);
         reg_acc = 0;
         if (vox_cell >= 0) atomic_add_u64(&a.vox[vox_cell], vox_acc);
         vox_cell = -1;
@@ -1001,8 +1009,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             if (a.det) {
                 for (int kd = 0; kd < a.ndet; ++kd) {
                     if (e7 >= a.det_rlo[kd] && e7 < a.det_rhi[kd]) {
-                        atomic_add_u64(&a.det[2 * kd], q0);
-                        atomic_add_u64(&a.det[2 * kd + 1], q1);
+                        const int64_t di = 2 * ((int64_t)ph.inst * a.ndet + kd);
+                        atomic_add_u64(&a.det[di], q0);
+                        atomic_add_u64(&a.det[di + 1], q1);
                     }
                 }
             }
@@ -1046,6 +1055,11 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     unsigned long long k = base + (unsigned long long)rank;
                     if (k < (unsigned long long)a.count) {
                         int64_t id = a.first + (int64_t)k;
+                        ph.inst = 0;
+                        if (a.inst_mat) {  // batched: k = instance * inst_count + index within it
+                            ph.inst = (int32_t)(k / (unsigned long long)a.inst_count);
+                            id = a.first + (int64_t)(k - (unsigned long long)ph.inst * (unsigned long long)a.inst_count);
+                        }
                         if constexpr (ADJ) {
                             const DetRec r = a.adj_list[k];
                             id = r.photon;
@@ -1070,7 +1084,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                         // PhotonClass::Init (.cl:344-389)
                         ph.v = mk(a.vstart[3 * dsel], a.vstart[3 * dsel + 1], a.vstart[3 * dsel + 2]);
                         ph.p = mk(a.pstart[0], a.pstart[1], a.pstart[2]);
-                        ph.w = 1.0f - a.rsp;
+                        ph.w = 1.0f - lmat[ph.inst * ntp1 + a.ntissue].x;
                         ph.sft = mk(0.0f, 0.0f, 0.0f);
                         ph.unfin = 0.0f;
                         ph.absorbed = 0.0f;
@@ -1115,7 +1129,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             bool act = alive && !search;
             if (act) {
                 if (!have_step) {
-                    const float4 cm = lmat[ph.curr];
+                    const float4 cm = lmat[ph.inst * ntp1 + ph.curr];
                     ballistic = (cm.y == 0.0f && cm.z == 0.0f);
                     if (!ballistic) {
                         if (ph.unfin == 0.0f) {
@@ -1295,7 +1309,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
 
 int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen)
 {
-    size_t lds = (size_t)(2 * a.nlds + kMaxTissues) * sizeof(float4);
+    size_t lds = (size_t)(2 * a.nlds + a.lmat_n) * sizeof(float4);
     hipError_t e;
     const void* fn;
     if (a.rng_mode == 1) lds += sizeof(Ran3Lds);
@@ -1363,6 +1377,23 @@ int launch_debug_copy(const void* src, void* dst, int64_t bytes, void* stream)
     if (n <= 0) return 0;
     hipLaunchKernelGGL(debug_copy_kernel, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const float4*)src,
                        (float4*)dst, n);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// Sets n 64-bit words to zero: clears tallies and the work counter inside
+// captured HIP graphs (a kernel node rather than a memset node).
+__global__ void zero_u64_kernel(unsigned long long* __restrict__ p, int64_t n)
+{
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
+}
+
+int launch_zero_u64(void* p, int64_t n, void* stream)
+{
+    if (n <= 0) return 0;
+    int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (unsigned long long*)p, n);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -1,0 +1,287 @@
+"""torch binding of the adjoint (lp_adjoint): annular detector signals as a
+differentiable function of every tissue's (mua, mus), and the fit of
+maternal and fetal arterial oxygen saturation to two-wavelength detector
+data with Adam.
+
+The transport runs in a child process (lightpropagation_amd.worker) that
+holds liblp.so; this process holds torch.  torch ships its own HIP runtime
+under the soname of the ROCm runtime liblp.so links, so the two are never
+loaded into one process.  Tensors cross as numpy arrays over a pipe; the
+work on the device is one forward launch and one replay launch per call.
+
+Estimator (the path-length or perturbation Monte Carlo estimator, evaluated
+on every detected path by replaying the photon's counter-based random
+stream): for a photon with exit weight W, path length L_t and n_t
+interactions in tissue t,
+    dW/dmua_t = -L_t W,    dW/dmus_t = (n_t / mus_t - L_t) W.
+The signals and gradients are means over the launched photons, so the
+backward pass is exact for the sample: grad_mua = g @ dS/dmua.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import torch
+
+from . import worker as _w
+
+_PKG_PARENT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+class WorkerError(RuntimeError):
+    pass
+
+
+class Simulator:
+    """A transport context in a child process.
+
+    scene: a scene dict or (builder, kwargs) with builder "maternal_fetal" or
+    "slab" of lightpropagation_amd.scenes (built in the child, which holds
+    the library).  det_lo/det_hi: annular detectors (exit radius in the
+    emitter frame).  jac_grid: (origin, size, n) of the voxel mua Jacobian.
+    """
+
+    def __init__(self, scene, seed, det_lo, det_hi, jac_grid=None, device=0, options=None, timeout=600):
+        r1, w1 = os.pipe()
+        r2, w2 = os.pipe()
+        env = dict(os.environ)
+        env["PYTHONPATH"] = _PKG_PARENT + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+        self.proc = subprocess.Popen([sys.executable, "-u", "-m", "lightpropagation_amd.worker", str(r1), str(w2)],
+                                     pass_fds=(r1, w2), env=env)
+        os.close(r1)
+        os.close(w2)
+        self._out = os.fdopen(w1, "wb")
+        self._in = os.fdopen(r2, "rb")
+        self.timeout = timeout
+        info = self.call("scene", spec=scene, seed=int(seed), det_lo=np.asarray(det_lo, np.float64),
+                         det_hi=np.asarray(det_hi, np.float64), jac_grid=jac_grid, device=device,
+                         options=options)
+        self.nmuasg = info["nmuasg"]
+        self.fetal_id = info["fetal_id"]
+        self.ndet = info["ndet"]
+        self.ntissue = len(self.nmuasg)
+
+    def call(self, cmd, **kw):
+        if self.proc is None:
+            raise WorkerError("simulator is closed")
+        try:
+            _w.send(self._out, (cmd, kw))
+            status, val = _w.recv(self._in)
+        except (EOFError, BrokenPipeError) as e:
+            rc = self.proc.poll()
+            raise WorkerError("worker process ended (exit status %s) during %r" % (rc, cmd)) from e
+        if status != "ok":
+            raise WorkerError(val)
+        return val
+
+    def optics(self, wavelength_nm, spo2_maternal, spo2_fetal):
+        """(n, mua, mus, g) of the maternal-fetal tissues (lpm_mf_optics)."""
+        return self.call("optics", wavelength_nm=float(wavelength_nm), spo2_maternal=float(spo2_maternal),
+                         spo2_fetal=float(spo2_fetal))
+
+    def adjoint(self, nmuasg, first, count, jac=False):
+        """Per-photon signals (ndet,), grad (ndet, ntissue, 2) and, with
+        jac, the voxel Jacobian (ndet, nz, ny, nx), as float64 numpy."""
+        return self.call("adjoint", first=int(first), count=int(count), jac=bool(jac),
+                         nmuasg=np.asarray(nmuasg, np.float32))
+
+    def forward(self, nmuasg, first, count):
+        """Per-photon detector signals of a forward run (no replay)."""
+        return self.call("run", first=int(first), count=int(count), nmuasg=np.asarray(nmuasg, np.float32))
+
+    def close(self):
+        if self.proc is None:
+            return
+        try:
+            self.call("close")
+        except WorkerError:
+            pass
+        for f in (self._out, self._in):
+            try:
+                f.close()
+            except OSError:
+                pass
+        try:
+            self.proc.wait(timeout=60)
+        except subprocess.TimeoutExpired:
+            self.proc.kill()
+            self.proc.wait()
+        self.proc = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DetectorSignals(torch.autograd.Function):
+    """S = signals(mua, mus): mean detector weight per launched photon of
+    photons [first, first + count), shape (ndet,).  mua, mus: (ntissue,)
+    tensors replacing the scene's columns (taken as float32, the transport's
+    type).  The same (first, count) in every call gives common random
+    numbers: S is then a deterministic function of (mua, mus)."""
+
+    @staticmethod
+    def forward(ctx, mua, mus, sim, first, count):
+        nm = np.array(sim.nmuasg, np.float32)
+        nm[:, 1] = mua.detach().to("cpu", torch.float32).numpy()
+        nm[:, 2] = mus.detach().to("cpu", torch.float32).numpy()
+        r = sim.adjoint(nm, first, count)
+        G = torch.from_numpy(np.ascontiguousarray(r["grad"])).to(device=mua.device, dtype=mua.dtype)
+        ctx.save_for_backward(G)
+        return torch.from_numpy(r["signals"]).to(device=mua.device, dtype=mua.dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        (G,) = ctx.saved_tensors
+        ga = g @ G[..., 0] if ctx.needs_input_grad[0] else None
+        gs = g @ G[..., 1] if ctx.needs_input_grad[1] else None
+        return ga, gs, None, None, None
+
+
+def detector_signals(mua, mus, sim, first, count):
+    return DetectorSignals.apply(mua, mus, sim, first, count)
+
+
+class SpO2Optics:
+    """mua of every tissue as a function of (maternal, fetal) SpO2 at one
+    wavelength: mua_t = a_t + b_t * SpO2 of the tissue's side, exact for the
+    maternal-fetal optics (haemoglobin absorption is linear in saturation);
+    a, b and the fetal mask come from three evaluations of lpm_mf_optics."""
+
+    def __init__(self, sim, wavelength_nm):
+        n00 = sim.optics(wavelength_nm, 0.0, 0.0)
+        n10 = sim.optics(wavelength_nm, 1.0, 0.0)
+        n01 = sim.optics(wavelength_nm, 0.0, 1.0)
+        self.wavelength = wavelength_nm
+        self.nmuasg = n00
+        a = n00[:, 1].astype(np.float64)
+        self.fetal = torch.from_numpy(n01[:, 1] != n00[:, 1])
+        b = np.where(self.fetal.numpy(), n01[:, 1], n10[:, 1]).astype(np.float64) - a
+        self.a = torch.from_numpy(a)
+        self.b = torch.from_numpy(b)
+        self.mus = torch.from_numpy(n00[:, 2].astype(np.float64))
+
+    def mua(self, spo2):
+        """spo2: tensor (2,) = (maternal, fetal)."""
+        s = torch.where(self.fetal, spo2[1], spo2[0])
+        return self.a.to(spo2) + self.b.to(spo2) * s
+
+
+def _residuals(x, sim, optics, measured, weights, first, count):
+    """sqrt(w_d) (log S_d - log M_d) over wavelengths and the detectors with
+    a measured signal and a nonzero weight."""
+    out = []
+    for i, (o, m) in enumerate(zip(optics, measured)):
+        wts = None if weights is None else weights[i]
+        m = torch.as_tensor(np.asarray(m, np.float64))
+        w = torch.ones_like(m) if wts is None else torch.as_tensor(np.asarray(wts, np.float64))
+        keep = (m > 0) & (w > 0)
+        S = detector_signals(o.mua(x), o.mus.clone(), sim, first, count)
+        if not bool(torch.all(S[keep] > 0)):
+            raise WorkerError("a detector with a measured signal received no photon; raise count")
+        out.append(torch.sqrt(w[keep]) * (torch.log(S[keep]) - torch.log(m[keep])))
+    return torch.cat(out)
+
+
+def _whitener(r, x, free):
+    """|r| V diag(lambda^-1/2) over the free coordinates, with
+    J^T J = V diag(lambda) V^T and J = dr/dx[free] from the graph of r (one
+    backward pass per residual, no transport).  In the coordinates z with
+    dx[free] = |r| V diag(lambda^-1/2) dz the linearised loss is
+    |r|^2 |z - z*|^2 with |z*| <= 1 and each coordinate along one principal
+    axis of the valley, so Adam's per-coordinate steps of about lr neither
+    stall along the weak axis nor throw it far past the optimum."""
+    J = torch.stack([torch.autograd.grad(r[i], x, retain_graph=True)[0][free] for i in range(len(r))])
+    ev, V = torch.linalg.eigh(J.T @ J)
+    ev = torch.clamp(ev, min=1e-12 * float(ev.max()) + 1e-300)
+    return float(torch.linalg.vector_norm(r.detach())) * (V @ torch.diag(ev.rsqrt()))
+
+
+def fit_spo2(sim, optics, measured, init=(0.8, 0.7), steps=60, lr=0.3, lr_end=0.002, first=0,
+             count=1_000_000, weights=None, free=(True, True), max_step=0.2, rewhiten=0, betas=(0.5, 0.9),
+             callback=None):
+    """Adam fit of (maternal, fetal) SpO2 to measured detector signals.
+
+    optics: list of SpO2Optics (one per wavelength); measured: list of
+    (ndet,) arrays, the signals per photon at each wavelength; weights: None
+    or one (ndet,) array per wavelength.  The loss is the sum over
+    wavelengths and detectors of w_d (log S_d - log M_d)^2 (w_d = 1 by
+    default; detectors with M_d = 0 or w_d = 0 are left out).  Every
+    evaluation runs photons [first, first + count) (common random numbers),
+    so the loss is a deterministic function of SpO2.  free selects the
+    fitted saturations (the others stay at init, e.g. a maternal SpO2 known
+    from a pulse oximeter).
+
+    The two saturations are seen with very different strength (the maternal
+    layers lie on every detected path, the fetal head only on the far
+    detectors' paths), so the loss is a long valley in SpO2 (the
+    Gauss-Newton matrix has condition numbers from hundreds to 10^5).  Adam
+    therefore steps in whitened coordinates z, SpO2 = x0 + |r| V L^-1/2 z,
+    where J^T J = V L V^T is the Gauss-Newton matrix and r the residuals at
+    x0 (J from the adjoint gradients already computed for the step).  Every
+    `rewhiten` steps (0: only at the start; None: never, plain Adam on SpO2)
+    x0 and the whitening are taken afresh and Adam restarts there.  A step
+    that would change a saturation by more than max_step is shortened to
+    max_step.  The learning rate decays exponentially from lr to lr_end
+    (constant when lr_end is None); the transport sees SpO2 clamped to
+    [0, 1], and the gradient passes the clamp unchanged (a coordinate past a
+    bound is pulled back when the optimum lies inside).  Returns (SpO2 numpy
+    (2,), history list of (loss, SpO2) at each step's evaluation point).
+
+    A two-saturation fit needs adjoint gradients whose fetal column is
+    accurate to better than its angle to the maternal column: the estimator
+    noise must not change the sign of the small eigenvalue of J^T J.
+    """
+    fidx = torch.tensor([i for i in range(2) if free[i]], dtype=torch.long)
+    if len(fidx) == 0:
+        raise ValueError("no free saturation")
+    k = len(fidx)
+    x0 = torch.tensor(init, dtype=torch.float64)
+    P = torch.eye(k, dtype=torch.float64)
+    gamma = 1.0 if lr_end is None else (lr_end / lr) ** (1.0 / max(steps - 1, 1))
+    z = opt = None
+    hist = []
+
+    def embed(zz):
+        dx = torch.zeros(2, dtype=torch.float64)
+        return dx.index_put((fidx,), P @ zz)
+
+    for it in range(steps):
+        if z is None:
+            z = torch.zeros(k, dtype=torch.float64, requires_grad=True)
+            opt = torch.optim.Adam([z], lr=lr, betas=betas)
+        xl = (x0 + embed(z.detach())).requires_grad_(True)
+        xe = xl + (torch.clamp(xl, 0.0, 1.0) - xl).detach()  # evaluated clamped, differentiated unclamped
+        r = _residuals(xe, sim, optics, measured, weights, first, count)
+        loss = (r * r).sum()
+        if rewhiten is not None and (it == 0 or (rewhiten > 0 and it % rewhiten == 0)):
+            P = _whitener(r, xl, fidx).detach()
+            x0 = xl.detach().clone()
+            z = torch.zeros(k, dtype=torch.float64, requires_grad=True)
+            opt = torch.optim.Adam([z], lr=lr, betas=betas)
+        (gx,) = torch.autograd.grad(loss, xl)
+        for g in opt.param_groups:
+            g["lr"] = lr * gamma ** it
+        opt.zero_grad()
+        z.grad = P.T @ gx[fidx]
+        z_prev = z.detach().clone()
+        opt.step()
+        with torch.no_grad():
+            dx = float(torch.max(torch.abs(P @ (z - z_prev))))
+            if dx > max_step:
+                z.copy_(z_prev + (z - z_prev) * (max_step / dx))
+        hist.append((float(loss.detach()), xe.detach().numpy().copy()))
+        if callback is not None:
+            callback(it, hist[-1])
+    with torch.no_grad():
+        x = torch.clamp(x0 + embed(z.detach()), 0.0, 1.0)
+    return x.numpy().copy(), hist

@@ -1,0 +1,28 @@
+"""The torch binding of the adjoint on the GPU: torch.autograd through the
+worker process against central differences in SpO2, and the Adam SpO2 fit.
+Each case runs in a child process (tests/torch_adjoint_gpu_case.py) that
+holds torch; that process starts the worker that holds liblp.so."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _run(case, timeout):
+    print("torch adjoint case", case, flush=True)
+    p = subprocess.run([sys.executable, "-u", os.path.join(HERE, "torch_adjoint_gpu_case.py"), case],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout)
+    print(p.stdout, flush=True)
+    assert p.returncode == 0 and "case passed" in p.stdout
+
+
+def test_autograd_against_central_differences():
+    _run("autograd", 300)
+
+
+def test_adam_spo2_fit():
+    _run("fit", 300)

@@ -120,7 +120,7 @@ int lp_set_source(lp_context *ctx, const float pstart[3], const float *vstart, c
 int lp_set_grid(lp_context *ctx, const float model_dims[3], const float det_grid[6]);
 
 /* Roulette chance, weight threshold, PixelPerUnit factor, seed, RNG mode and
- * a per-photon step cap (<= 0: the default of 10^6).  Replaces
+ * a per-photon step cap (<= 0: the default of 10^6; at most 2^31 - 1).  Replaces
  * SetChanceForRoulette / SetWeightThreshold / SetPixelPerUnitFactor
  * (cpp:405-425) and the PSNR seeding of setup() (cpp:1893-1899). */
 int lp_set_params(lp_context *ctx, float chance, float w_th, float factor, uint64_t seed, int32_t rng_mode,

@@ -33,10 +33,15 @@ typedef struct lpm_mf_params {
     double t_scalp, t_skull;        /* fetal layer thicknesses (0.1, 0.3)      */
     int32_t freq_maternal;          /* icosphere frequency of maternal spheres */
     int32_t freq_fetal;             /* icosphere frequency of fetal spheres    */
+    /* optional fetal torso: an ellipsoid of fetal soft tissue (tissue 5, the
+     * scalp's optics) in the amniotic fluid; freq_torso 0 = none */
+    int32_t freq_torso;
+    double torso_axes[3];           /* semi-axes along x, y, z (4.5, 4.0, 6.0)  */
+    double torso_center[3];         /* centre (0, 0, 18.0): below the head      */
 } lpm_mf_params;
 
 /* Fills *p with the defaults above and a frequency of 38 for all seven
- * spheres (202 160 faces). */
+ * spheres (202 160 faces), no torso. */
 void lpm_mf_default(lpm_mf_params *p);
 int64_t lpm_mf_num_triangles(const lpm_mf_params *p);
 int lpm_mf_build(const lpm_mf_params *p, float *tri, float *nrm, int32_t *med, int64_t ntri);

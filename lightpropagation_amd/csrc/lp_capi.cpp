@@ -514,7 +514,7 @@ int lp_set_params(lp_context* c, float chance, float w_th, float factor, uint64_
     c->factor = factor;
     c->seed = seed;
     c->rng_mode = rng_mode;
-    c->max_steps = max_steps > 0 ? max_steps : 1000000;
+    c->max_steps = max_steps > 0 ? std::min<int64_t>(max_steps, 0x7fffffff) : 1000000;  // steps count in 32 bits
     return LP_OK;
 }
 

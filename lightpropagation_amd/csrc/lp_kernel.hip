@@ -524,8 +524,11 @@ __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned l
 // the transport kernel
 // ---------------------------------------------------------------------------
 struct Photon {
-    f3 p, v, nv, sft;
-    float w, unfin, absorbed;
+    f3 p, v;
+    float w, unfin;
+    // last boundary normal as (face id << 1) | near-axis override, -1 for the
+    // initial (0, 0, -1); the exit angle recomputes it from the face record
+    int32_t nvc;
     // free region: the ball of radius safe around anc, cut by the
     // half-space dot(pu, x) > pc (pu = 0, pc = -1: no cut); no face meets
     // the region, p lies inside it, and it is convex, so a step whose end
@@ -535,7 +538,7 @@ struct Photon {
     int32_t ref, curr;
     int32_t fetal, roulette;
     int32_t inst;  // instance of a batched launch (0 otherwise)
-    int64_t steps;
+    uint32_t steps;
     int64_t k;  // index within this run
 };
 
@@ -697,6 +700,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     bool search = false;     // the pending step needs a face search
     bool by_list = false;    // ... from the near-face lists (else from the BVH)
 
+    // weights recorded at the exit (R, fetal R, T): set by the boundary event
+    // that ends the photon, read by the epilogue in the same iteration
+    f3 xsft = mk(0.0f, 0.0f, 0.0f);
     int32_t adj_det = 0;  // adjoint: the detector and R weight of the replayed photon
     float adj_w = 0.0f;
     // adjoint: one move of length s from p0 in the current tissue
@@ -718,9 +724,12 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         ph.p = add(ph.p, mul(s, ph.v));
         if constexpr (ADJ) adj_step(p_old, s, true);
         ph.ref = -1;
-        if (cm.y > 0.0f && cm.z > 0.0f) {  // Absorb (.cl:841-847)
-            ph.absorbed = ph.w * (cm.y / (cm.y + cm.z));
-            ph.w -= ph.absorbed;
+        // Absorb (.cl:841-847); skipped only in a start tissue with mua or
+        // mus 0, which a photon never re-enters and where nothing was absorbed
+        float absorbed = 0.0f;
+        if (cm.y > 0.0f && cm.z > 0.0f) {
+            absorbed = ph.w * (cm.y / (cm.y + cm.z));
+            ph.w -= absorbed;
         }
         float xt = rng.next();
         float xp = rng.next();
@@ -765,7 +774,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             // absorption tallies at the new position (replaces RecordAbsorbtionRZ, .cl:254-271)
         const f3 d = sub(ph.p, POI);
         const f3 c = mk(dot(eX, d), dot(eY, d), dot(eZ, d));
-        unsigned long long q = fx32(ph.absorbed);
+        unsigned long long q = fx32(absorbed);
         reg_acc += q;
         if (a.arz) {
             float rho = __builtin_sqrtf(c.x * c.x + c.y * c.y) * sign_cl(c.x) / a.det_grid[3];
@@ -774,7 +783,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             if (ir >= -a.dimR && ir <= a.dimR && iz >= 0 && iz <= a.dimz) {
                 int64_t cell = (int64_t)iz * a.arz_nr + (ir + a.dimR);
                 atomic_add_u64(&a.arz[cell], q);
-                atomic_add_u64(&a.phirz[cell], fx24(ph.absorbed / (cm.y == 0.0f ? 1.0f : cm.y)));
+                atomic_add_u64(&a.phirz[cell], fx24(absorbed / (cm.y == 0.0f ? 1.0f : cm.y)));
             } else {
                 if (q) atomicAdd(&wc[wArzOut], q);
             }
@@ -814,13 +823,14 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         } else {
             float x = rng.next();
             int32_t in_id = h.med & 0xffff, out_id = (h.med >> 16) & 0xffff;
-            ph.nv = normalize(mk(a.nrm[3 * h.id], a.nrm[3 * h.id + 1], a.nrm[3 * h.id + 2]));
+            f3 nv = normalize(mk(a.nrm[3 * h.id], a.nrm[3 * h.id + 1], a.nrm[3 * h.id + 2]));
             int32_t nx_id = (ph.curr == out_id) ? in_id : out_id;
             const float4 nm = lmat[ph.inst * ntp1 + nx_id];
-            if (__builtin_fabsf(__builtin_sqrtf(ph.p.x * ph.p.x + ph.p.y * ph.p.y)) < 0.01f)
-                ph.nv = mul(sign_cl(dot(ph.nv, eZr)), eZ);
+            const bool axis = __builtin_fabsf(__builtin_sqrtf(ph.p.x * ph.p.x + ph.p.y * ph.p.y)) < 0.01f;
+            if (axis) nv = mul(sign_cl(dot(nv, eZr)), eZ);
+            ph.nvc = (h.id << 1) | (axis ? 1 : 0);
             // TransmitOrReflect (.cl:921-1018)
-            float cos_in = dot(ph.nv, ph.v);
+            float cos_in = dot(nv, ph.v);
             float abs_cos = __builtin_fabsf(cos_in);
             float n1 = cm.x, n2 = nm.x;
             float cos_crit = 0.0f;
@@ -852,21 +862,21 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     r = 0.5f * (sam * sam) * ((cam * cam) + (cap * cap)) / ((sap * sap) * (cam * cam));
                 }
             }
-            f3 ey = normalize(cross(ph.nv, cross(ph.nv, ph.v)));
+            f3 ey = normalize(cross(nv, cross(nv, ph.v)));
             int32_t new_id;
             float m_a, m_s;
             if (x > r) {  // Transmit (.cl:506-529)
                 float sin_t = __builtin_sqrtf(clamp0(1 - cos_in * cos_in)) * n1 / n2;
                 float cos_t = -__builtin_sqrtf(clamp0(1 - sin_t * sin_t));
                 if (cos_in > 0.0f) cos_t = __builtin_fabsf(cos_t);
-                ph.v = normalize(sub(mul(cos_t, ph.nv), mul(sin_t, ey)));
+                ph.v = normalize(sub(mul(cos_t, nv), mul(sin_t, ey)));
                 new_id = (cos_in > 0) ? out_id : in_id;
                 m_a = nm.y;
                 m_s = nm.z;
                 kind = 1;
             } else {  // Reflect (.cl:532-549)
                 float sin_r = __builtin_sqrtf(clamp0(1 - cos_in * cos_in));
-                ph.v = neg(normalize(add(mul(cos_in, ph.nv), mul(sin_r, ey))));
+                ph.v = neg(normalize(add(mul(cos_in, nv), mul(sin_r, ey))));
                 new_id = ph.curr;
                 m_a = cm.y;
                 m_s = cm.z;
@@ -878,9 +888,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 float dist = __builtin_fabsf(dot(eZ, sub(ph.p, POI)));
                 float wr = ph.w * (1 - lmat[ph.inst * ntp1 + a.ntissue].x);  // (1 - Rsp)
                 bool trans = dist > a.dist_min;
-                ph.sft.z = trans ? wr : ph.sft.z;
-                ph.sft.x = trans ? ph.sft.x : wr;
-                ph.sft.y = (!trans && ph.fetal) ? wr : 0.0f;
+                xsft.z = trans ? wr : 0.0f;
+                xsft.x = trans ? 0.0f : wr;
+                xsft.y = (!trans && ph.fetal) ? wr : 0.0f;
                 ph.w = 0.0f;
             }
             if (new_id != ph.curr) {
@@ -910,8 +920,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             }
         }
         ph.steps++;
-        if (TRACE && ph.steps <= a.trace_max) {
-            float* t = a.trace + ((int64_t)ph.k * a.trace_max + (ph.steps - 1)) * 12;
+        if (TRACE && ph.steps <= (uint32_t)a.trace_max) {
+            float* t = a.trace + ((int64_t)ph.k * a.trace_max + ((int64_t)ph.steps - 1)) * 12;
             t[0] = (float)kind;
             t[1] = (float)hid;
             t[2] = (float)ph.curr;
@@ -925,7 +935,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             t[10] = s;
             t[11] = (float)(TRACE ? rng.ndraw : 0);
         }
-        if (a.max_steps > 0 && ph.steps >= a.max_steps && ph.w > 0.0f) {
+        if (a.max_steps > 0 && (int64_t)ph.steps >= a.max_steps && ph.w > 0.0f) {
             wcount(wCap);
             ph.w = 0.0f;
         }
@@ -939,7 +949,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             wcount(wNan);
         if constexpr (ADJ) {
             // the replay must end as the forward photon did
-            if (ph.sft.x != adj_w) wcount(wMismatch);
+            if (xsft.x != adj_w) wcount(wMismatch);
             const float W = adj_w;
             for (int32_t t = 0; t < a.ntissue; ++t) {
                 const float L = adjL[t * kBlock + tid];
@@ -962,15 +972,21 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         const f3 dq = sub(ph.p, POI);
         const f3 dn = sub(POI, ph.p);
         const float e0 = dot(eX, dq), e1 = dot(eY, dq), e2 = dot(eZ, dq);
-        const float e3 = ph.sft.x, e4 = ph.sft.y, e5 = ph.sft.z;
-        const float e6 = __builtin_fabsf(acosf_lp(dot(ph.v, ph.nv)));
+        const float e3 = xsft.x, e4 = xsft.y, e5 = xsft.z;
+        f3 nv = mk(0.0f, 0.0f, -1.0f);  // the last boundary normal, as the boundary event computed it
+        if (ph.nvc >= 0) {
+            const int32_t id = ph.nvc >> 1;
+            nv = normalize(mk(a.nrm[3 * id], a.nrm[3 * id + 1], a.nrm[3 * id + 2]));
+            if (ph.nvc & 1) nv = mul(sign_cl(dot(nv, eZr)), eZ);
+        }
+        const float e6 = __builtin_fabsf(acosf_lp(dot(ph.v, nv)));
         const float e7 = __builtin_sqrtf((dn.x * dn.x + dn.y * dn.y) + dn.z * dn.z);
         if (a.exit_rec) {
             float4* o = reinterpret_cast<float4*>(a.exit_rec + 8 * ph.k);
             o[0] = make_float4(e0, e1, e2, e3);
             o[1] = make_float4(e4, e5, e6, e7);
         }
-        if (TRACE) a.trace_n[ph.k] = (int32_t)(ph.steps < a.trace_max ? ph.steps : a.trace_max);
+        if (TRACE) a.trace_n[ph.k] = (int32_t)(ph.steps < (uint32_t)a.trace_max ? ph.steps : (uint32_t)a.trace_max);
         if (e3 != 0.0f || e4 != 0.0f || e5 != 0.0f) {
             unsigned long long q0 = fx32(e3), q1 = fx32(e4), q2 = fx32(e5);
             if (a.sft) {
@@ -1085,13 +1101,11 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                         ph.v = mk(a.vstart[3 * dsel], a.vstart[3 * dsel + 1], a.vstart[3 * dsel + 2]);
                         ph.p = mk(a.pstart[0], a.pstart[1], a.pstart[2]);
                         ph.w = 1.0f - lmat[ph.inst * ntp1 + a.ntissue].x;
-                        ph.sft = mk(0.0f, 0.0f, 0.0f);
                         ph.unfin = 0.0f;
-                        ph.absorbed = 0.0f;
                         ph.fetal = 0;
                         ph.ref = -1;
                         ph.curr = 0;
-                        ph.nv = mk(0.0f, 0.0f, -1.0f);
+                        ph.nvc = -1;
                         ph.roulette = 1;
                         ph.steps = 0;
                         ph.safe = 0.0f;
@@ -1125,6 +1139,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             Hit h;
             h.id = -1;
             h.t = 10000.0f;
+            xsft = mk(0.0f, 0.0f, 0.0f);
             if (alive) rng.top_up();
             bool act = alive && !search;
             if (act) {

@@ -172,12 +172,20 @@ void lpm_mf_default(lpm_mf_params* p)
     p->t_skull = 0.3;
     p->freq_maternal = 38;
     p->freq_fetal = 38;
+    p->freq_torso = 0;
+    p->torso_axes[0] = 4.5;
+    p->torso_axes[1] = 4.0;
+    p->torso_axes[2] = 6.0;
+    p->torso_center[0] = 0.0;
+    p->torso_center[1] = 0.0;
+    p->torso_center[2] = 18.0;
 }
 
 int64_t lpm_mf_num_triangles(const lpm_mf_params* p)
 {
-    if (!p || p->freq_maternal < 1 || p->freq_fetal < 1) return -1;
-    return 4 * 20LL * p->freq_maternal * p->freq_maternal + 3 * 20LL * p->freq_fetal * p->freq_fetal;
+    if (!p || p->freq_maternal < 1 || p->freq_fetal < 1 || p->freq_torso < 0) return -1;
+    return 4 * 20LL * p->freq_maternal * p->freq_maternal + 3 * 20LL * p->freq_fetal * p->freq_fetal +
+           20LL * p->freq_torso * p->freq_torso;
 }
 
 int lpm_mf_build(const lpm_mf_params* p, float* tri, float* nrm, int32_t* med, int64_t ntri)
@@ -200,6 +208,23 @@ int lpm_mf_build(const lpm_mf_params* p, float* tri, float* nrm, int32_t* med, i
     k = emit_sphere(gf, cf, p->r_head, 5, 4, tri, nrm, med, k);
     k = emit_sphere(gf, cf, p->r_head - p->t_scalp, 6, 5, tri, nrm, med, k);
     k = emit_sphere(gf, cf, p->r_head - p->t_scalp - p->t_skull, 7, 6, tri, nrm, med, k);
+    if (p->freq_torso > 0) {
+        // the torso must lie in the cavity and clear of the head (checked
+        // with its largest semi-axis, a bounding sphere)
+        const V3 ct = {p->torso_center[0], p->torso_center[1], p->torso_center[2]};
+        const double amax = std::max(p->torso_axes[0], std::max(p->torso_axes[1], p->torso_axes[2]));
+        const double dm = std::sqrt(dt(sub(ct, cm), sub(ct, cm))), dh = std::sqrt(dt(sub(ct, cf), sub(ct, cf)));
+        if (!(p->torso_axes[0] > 0 && p->torso_axes[1] > 0 && p->torso_axes[2] > 0 && dm + amax < r_cav &&
+              dh > p->r_head + amax))
+            return 3;
+        Geodesic gt = make_geodesic(p->freq_torso);
+        for (auto& F : gt.faces) {  // the unit sphere scaled per axis: an ellipsoid, still wound outward
+            V3 q[3];
+            for (int v = 0; v < 3; ++v)
+                for (int x = 0; x < 3; ++x) q[v][x] = ct[x] + p->torso_axes[x] * gt.pts[F[v]][x];
+            put_tri(tri, nrm, med, k++, q[0], q[1], q[2], 5, 4);
+        }
+    }
     return k == ntri ? 0 : 1;
 }
 

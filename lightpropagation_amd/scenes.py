@@ -20,7 +20,8 @@ class MFParams(C.Structure):
     _fields_ = [("r_maternal", C.c_double), ("t_skin", C.c_double), ("t_fat", C.c_double),
                 ("t_uterus", C.c_double), ("fetal_depth", C.c_double), ("r_head", C.c_double),
                 ("t_scalp", C.c_double), ("t_skull", C.c_double), ("freq_maternal", C.c_int32),
-                ("freq_fetal", C.c_int32)]
+                ("freq_fetal", C.c_int32), ("freq_torso", C.c_int32), ("torso_axes", C.c_double * 3),
+                ("torso_center", C.c_double * 3)]
 
 
 def _chk(rc, what):
@@ -48,6 +49,8 @@ def mf_mesh(**kw):
     p = MFParams()
     lib().lpm_mf_default(C.byref(p))
     for k, v in kw.items():
+        if k in ("torso_axes", "torso_center"):
+            v = (C.c_double * 3)(*v)
         setattr(p, k, v)
     n = lib().lpm_mf_num_triangles(C.byref(p))
     tri = np.zeros((n, 9), np.float32)

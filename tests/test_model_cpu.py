@@ -1,0 +1,86 @@
+"""The maternal-fetal mesh generator (lpm_mf_build) on the CPU: face counts,
+every surface closed (each edge used once in each direction), normals
+pointing out of the tissue they bound, the tissue tags, and the optional
+fetal torso ellipsoid inside the amniotic cavity and clear of the head."""
+import numpy as np
+import pytest
+
+from lightpropagation_amd import scenes as S
+
+
+def _surfaces(tri, med):
+    """Groups faces by (inner, outer) tag in the generator's emission order."""
+    out = {}
+    for k in range(len(tri)):
+        out.setdefault((int(med[k, 0]), int(med[k, 1])), []).append(k)
+    return out
+
+
+def _closed(tri_sub):
+    """Each directed edge appears exactly once and its reverse exactly once."""
+    v = np.round(tri_sub.reshape(-1, 3, 3).astype(np.float64), 6)
+    edges = {}
+    for f in v:
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            e = (tuple(f[a]), tuple(f[b]))
+            edges[e] = edges.get(e, 0) + 1
+    return all(n == 1 and edges.get((e[1], e[0]), 0) == 1 for e, n in edges.items())
+
+
+@pytest.mark.parametrize("torso", [0, 6])
+def test_mesh_counts_tags_closure_and_normals(torso):
+    tri, nrm, med, p = S.mf_mesh(freq_maternal=5, freq_fetal=4, freq_torso=torso)
+    assert len(tri) == 4 * 20 * 25 + 3 * 20 * 16 + 20 * torso * torso
+    surf = _surfaces(tri, med)
+    want = {(1, 0), (2, 1), (3, 2), (4, 3), (5, 4), (6, 5), (7, 6)}
+    assert set(surf) == want
+    cm = np.array([0.0, 0.0, p.r_maternal])
+    ch = np.array([0.0, 0.0, p.fetal_depth + p.r_head])
+    for tag, ks in surf.items():
+        f = tri[ks].reshape(-1, 3, 3).astype(np.float64)
+        if tag == (5, 4) and torso:
+            # the head's outer sphere and the torso share the tag: split them
+            head = np.linalg.norm(f.mean(1) - ch, axis=1) < p.r_head + 0.5
+            parts = [f[head], f[~head]]
+        else:
+            parts = [f]
+        for part in parts:
+            assert _closed(part.reshape(len(part), 9)), tag
+        # normals point away from the centre of the shape they bound (outward)
+        c = f.mean(1)
+        centre = cm if tag[0] <= 4 else ch
+        n = nrm[ks].astype(np.float64)
+        if tag == (5, 4) and torso:
+            tc = np.array(p.torso_center[:])
+            centre = np.where(np.linalg.norm(c - ch, axis=1, keepdims=True) < p.r_head + 0.5, ch, tc)
+        assert np.all(np.einsum("ij,ij->i", n, c - centre) > 0), tag
+        assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-6)
+
+
+def test_torso_lies_in_the_cavity_and_clear_of_the_head():
+    tri, nrm, med, p = S.mf_mesh(freq_maternal=4, freq_fetal=4, freq_torso=12)
+    f = tri[-20 * 144:].reshape(-1, 3).astype(np.float64)
+    assert np.all(med[-20 * 144:] == [5, 4])
+    r_cav = p.r_maternal - p.t_skin - p.t_fat - p.t_uterus
+    assert np.all(np.linalg.norm(f - [0, 0, p.r_maternal], axis=1) < r_cav)
+    assert np.all(np.linalg.norm(f - [0, 0, p.fetal_depth + p.r_head], axis=1) > p.r_head)
+    # the vertices lie on the ellipsoid
+    q = (f - np.array(p.torso_center[:])) / np.array(p.torso_axes[:])
+    assert np.allclose(np.linalg.norm(q, axis=1), 1.0, atol=1e-5)
+
+
+def test_misplaced_torso_is_rejected():
+    with pytest.raises(ValueError):
+        S.mf_mesh(freq_maternal=4, freq_fetal=4, freq_torso=4, torso_center=(0.0, 0.0, 8.0))  # in the head
+    with pytest.raises(ValueError):
+        S.mf_mesh(freq_maternal=4, freq_fetal=4, freq_torso=4, torso_center=(0.0, 0.0, 26.0))  # past the cavity
+
+
+def test_large_mesh_size():
+    # the 2e6-face mesh: frequency 118 spheres and a frequency-40 torso
+    import ctypes as C
+    p = S.MFParams()
+    S.lib().lpm_mf_default(C.byref(p))
+    p.freq_maternal = p.freq_fetal = 118
+    p.freq_torso = 40
+    assert S.lib().lpm_mf_num_triangles(C.byref(p)) == 7 * 20 * 118 * 118 + 20 * 40 * 40 == 1_981_360

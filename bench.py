@@ -240,6 +240,45 @@ def batch_c2(photons=1_000_000, tune=True):
             "photons_per_s": total_ph / total_s, "photons": total_ph, "seconds": total_s, "geometries": out}
 
 
+def large_c4(photons=100_000_000):
+    """The large workload: frequency-118 shells plus a frequency-40 fetal
+    torso ellipsoid (1 981 360 faces), 940 nm, a 512^3 int64 absorption grid
+    (1 GiB) and 8 detectors; photons/s of the forward pass, then the adjoint
+    (forward + replay, 7 tissues x (mua, mus)) of the same photons."""
+    import lightpropagation_amd as lp
+    from lightpropagation_amd import scenes
+    sc = scenes.maternal_fetal(wavelength_nm=940.0, freq_maternal=118, freq_fetal=118, freq_torso=40)
+    ctx = lp.Context(0)
+    t0 = time.perf_counter()
+    ctx.set_scene(sc, seed=0x5EED0004)
+    lo, hi = detectors()
+    ctx.set_detectors(lo, hi)
+    ctx.set_voxel_grid([-12.8, -12.8, 0.0], [0.05, 0.05, 0.05], [512, 512, 512])
+    ctx.set_tallies(sum(1 << lp.TALLY[k] for k in ("sft", "region", "vox", "det", "misc")))
+    ctx.run(1 << 40, 100_000)  # builds the BVH and near-face lists, uploads
+    setup_s = time.perf_counter() - t0
+    ctx.reset()
+    t0 = time.perf_counter()
+    ctx.run(0, photons)
+    fwd_s = time.perf_counter() - t0
+    st = ctx.stats()
+    m = ctx.tallies()["misc"]
+    print("large: %d faces, %d photons in %.1f s" % (len(sc["tri"]), photons, fwd_s), file=sys.stderr, flush=True)
+    ctx.set_tallies(1 << lp.TALLY["det"] | 1 << lp.TALLY["misc"])
+    ctx.reset()
+    t0 = time.perf_counter()
+    r = ctx.adjoint(0, photons)
+    adj_s = time.perf_counter() - t0
+    sa = ctx.stats()
+    ctx.close()
+    return {"workload": "1 981 360 faces (shells + fetal torso), 940 nm, 512^3 int64 grid + 8 detectors, %d photons"
+                        % photons,
+            "setup_s": setup_s, "photons_per_s": photons / fwd_s, "kernel_ms": st.kernel_ms,
+            "steps_per_photon": m["steps"] / photons, "device_bytes": st.device_bytes,
+            "adjoint_s": adj_s, "adjoint_forward_ms": sa.kernel_ms, "adjoint_replay_ms": sa.replay_ms,
+            "adjoint_pairs": r["pairs"], "adjoint_over_forward": adj_s / fwd_s}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +296,8 @@ def main():
     ap.add_argument("--adjoint-photons", type=int, default=10_000_000)
     ap.add_argument("--c2", type=int, default=0, metavar="PHOTONS",
                     help="also run the batched 128-instance workload with this many photons per instance")
+    ap.add_argument("--c4", type=int, default=0, metavar="PHOTONS",
+                    help="also run the large 2e6-face workload with this many photons")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run a plain copy of this many MiB (reference for profiler byte counters)")
     args = ap.parse_args()
@@ -355,6 +396,8 @@ def main():
         out["adjoint"] = adjoint_vs_fd(sc, photons=args.adjoint_photons)
     if args.c2:
         out["batch_c2"] = batch_c2(args.c2)
+    if args.c4:
+        out["large_c4"] = large_c4(args.c4)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
     print(json.dumps(out))

@@ -1164,6 +1164,26 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 const f3 q = add(ph.p, mul(stepsize, ph.v));
                 const f3 e = sub(q, ph.anc);
                 freed = !ballistic && dot(e, e) < ph.safe * ph.safe && dot(ph.pu, q) > ph.pc;
+                if (!freed && !ballistic && a.near_cells) {
+                    // else the ball of the near-grid cell's clearance around p:
+                    // no face lies within the clearance of any point the cell
+                    // lookup places in the cell
+                    const int64_t c = near_cell(a, ph.p);
+                    if (c >= 0) {
+                        const float cl = __uint_as_float(a.near_cells[c].clear_bits);
+                        const float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
+                                                __builtin_fabsf(ph.p.z));
+                        const float sc = cl - (a.safe_margin + 1e-5f * (cl + rad));
+                        const f3 eq = sub(q, ph.p);
+                        if (sc > 0.0f && dot(eq, eq) < sc * sc) {
+                            ph.anc = ph.p;
+                            ph.safe = sc;
+                            ph.pu = mk(0.0f, 0.0f, 0.0f);
+                            ph.pc = -1.0f;
+                            freed = true;
+                        }
+                    }
+                }
                 if (freed || (!ballistic && a.safety && stepsize < safety_at(a, ph.p))) {
                     go = true;
                     wcount(wSkip);
@@ -1171,7 +1191,10 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     search = true;
                     const bool finite_p =
                         __builtin_isfinite(ph.p.x) && __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
-                    by_list = a.near_cells && !ballistic && stepsize <= a.near_r0 && finite_p;
+                    // the near-face lists answer a step of at most r0, and a
+                    // ballistic lane's flight when its face lies within r0
+                    // (a miss there falls back to the BVH)
+                    by_list = a.near_cells && finite_p && (ballistic || stepsize <= a.near_r0);
                 }
             }
             const int nlive = __popcll(__ballot(alive));
@@ -1188,12 +1211,23 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 const bool part = search && by_list;
                 float d1sq, d2sq, unlisted;
                 int32_t i1;
-                const bool want = ph.ref < 0;
+                const bool want = ph.ref < 0 && !ballistic;
                 Hit hl;
                 uint32_t pairs;
-                near_batch(a, part, ph.p, ph.v, ph.ref, stepsize, want, hl, d1sq, d2sq, i1, unlisted, pairs);
+                near_batch(a, part, ph.p, ph.v, ph.ref, ballistic ? a.near_r0 : stepsize, want, hl, d1sq, d2sq, i1,
+                           unlisted, pairs);
                 if (lane == 0) atomicAdd(&wc[wLst], (unsigned long long)pairs);
-                if (part) {
+                if (part && ballistic) {
+                    if (hl.id >= 0) {  // the nearest face ahead, within r0
+                        h = hl;
+                        stepsize = h.t;
+                        bnd = true;
+                        search = false;
+                        go = true;
+                    } else {
+                        by_list = false;  // farther than r0: the BVH
+                    }
+                } else if (part) {
                     h = hl;
                     if (want) {
                         const float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
@@ -1238,7 +1272,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 // test at t <= s lies within s + margin of p)
                 const bool finite_p =
                     __builtin_isfinite(ph.p.x) && __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
-                bool dq = sb && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && finite_p;
+                // a distance query capped below the step cannot free it
+                bool dq = sb && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && finite_p &&
+                          stepsize < fminf(a.dist_cap, a.dist_cap_steps * stepsize);
                 if (__any(dq)) {
                     if (dq) {
                         float cap = fminf(a.dist_cap, a.dist_cap_steps * stepsize);

@@ -202,7 +202,8 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
 #define LP_OPT_DIST_CAP 3       /* free-radius query: search cap in length units (0 = off) */
 #define LP_OPT_DIST_CAP_STEPS 4 /* free-radius query: cap in units of the step length */
 #define LP_OPT_BLOCKS_PER_CU 5  /* work-groups per CU when GRID_BLOCKS is automatic */
-#define LP_OPT_FREE_ITERS 6     /* search-free steps a lane may take per round      */
+#define LP_OPT_FREE_ITERS 6     /* wavefront iterations per round (search batches are
+                                   forced and finished lanes refilled at its end)  */
 #define LP_OPT_WAVES_PER_SIMD 7 /* register budget of the kernel variant: 1 (compiler's
                                    choice), 4 (default), 5 or 6 waves per SIMD     */
 #define LP_OPT_NEAR_R0 8        /* near-face lists: radius r0 (0 = off, default 0.05) */
@@ -215,6 +216,9 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
                                    wavefront have finished (1..64)                 */
 #define LP_OPT_GRAPH 13         /* lp_run_batch through a captured HIP graph (1, default)
                                    or eager launches (0)                           */
+#define LP_OPT_CLEAR_STEP 14    /* check a step that leaves its free region against the
+                                   near-grid cell clearance when it is at least this
+                                   long (0, default: every step; < 0: never)        */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
 
 /* ---- batched launch (replaces ExecKernel's loop of 6 400-photon launches,

@@ -62,6 +62,7 @@ struct lp_context {
     float search_min = 1.0f;
     float list_min = 0.25f;
     int32_t refill_min = 16;
+    float clear_step = 0.0f;
     int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
     float jac_origin[3] = {0, 0, 0}, jac_size[3] = {1, 1, 1};
@@ -300,6 +301,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.free_iters = c->free_iters;
     a.search_min = c->search_min;
     a.list_min = c->list_min;
+    a.clear_step = c->clear_step;
     a.refill_min = c->refill_min;
     a.waves = c->waves;
     a.safe_margin = 2e-6f * c->bvh_extent;
@@ -613,6 +615,10 @@ int lp_set_option(lp_context* c, int32_t key, double v)
         if (!(v >= 0 && v < 1e6)) return fail(c, LP_EINVAL, "near cell must be >= 0");
         if ((float)v != c->near_cell) c->mesh_dirty = true;
         c->near_cell = (float)v;
+        return LP_OK;
+    case LP_OPT_CLEAR_STEP:
+        if (!(v > -1e30 && v < 1e30)) return fail(c, LP_EINVAL, "clearance step out of range");
+        c->clear_step = (float)v;
         return LP_OK;
     case LP_OPT_GRAPH:
         if (!(v == 0 || v == 1)) return fail(c, LP_EINVAL, "graph option must be 0 or 1");

@@ -115,9 +115,10 @@ struct KArgs {
     float dist_cap;        // per-photon free-radius search: cap (0 = off)
     float dist_cap_steps;  // ... and cap in units of the current step length
     float safe_margin;     // absolute rounding margin of the free radius
-    int32_t free_iters;    // search-free steps a lane may take per round
+    int32_t free_iters;    // wavefront iterations per round (batches forced, lanes refilled at its end)
     float search_min;      // run a BVH search batch when this fraction of live lanes waits for one
     float list_min;        // run a near-list search batch when this fraction of live lanes waits for one
+    float clear_step;      // cell-clearance free check for steps of at least this length (< 0: off)
     int32_t refill_min;    // end the round once this many lanes of a wavefront have finished
     int32_t waves;         // occupancy target of the kernel variant (1, 4, 5, 6)
     float nmuasg[4 * kMaxTissues];

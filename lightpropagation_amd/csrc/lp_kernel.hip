@@ -1164,7 +1164,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 const f3 q = add(ph.p, mul(stepsize, ph.v));
                 const f3 e = sub(q, ph.anc);
                 freed = !ballistic && dot(e, e) < ph.safe * ph.safe && dot(ph.pu, q) > ph.pc;
-                if (!freed && !ballistic && a.near_cells) {
+                if (!freed && !ballistic && a.near_cells && a.clear_step >= 0.0f && stepsize >= a.clear_step) {
                     // else the ball of the near-grid cell's clearance around p:
                     // no face lies within the clearance of any point the cell
                     // lookup places in the cell
@@ -1322,8 +1322,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     alive = false;
                 }
             }
-            // end the round at the cap, or once enough lanes have finished
-            // and photons are left to start
+            // end the round at the cap (finished lanes are refilled then), or
+            // once enough lanes have finished and photons are left to start
             if (at_cap || (!exhausted && 64 - __popcll(__ballot(alive)) >= a.refill_min)) break;
         }
     }

@@ -298,6 +298,8 @@ def main():
                     help="also run the batched 128-instance workload with this many photons per instance")
     ap.add_argument("--c4", type=int, default=0, metavar="PHOTONS",
                     help="also run the large 2e6-face workload with this many photons")
+    ap.add_argument("--fit", action="store_true",
+                    help="also fit fetal SpO2 to synthetic two-wavelength data (torch, in a child process)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run a plain copy of this many MiB (reference for profiler byte counters)")
     args = ap.parse_args()
@@ -398,6 +400,12 @@ def main():
         out["batch_c2"] = batch_c2(args.c2)
     if args.c4:
         out["large_c4"] = large_c4(args.c4)
+    if args.fit:
+        # torch and liblp.so bring two HIP runtimes: the fit runs in a process of its own
+        import subprocess
+        p = subprocess.run([sys.executable, "-m", "lightpropagation_amd.spo2_fit"], cwd=ROOT, stdout=subprocess.PIPE,
+                           text=True, timeout=1200)
+        out["spo2_fit"] = json.loads(p.stdout.strip().splitlines()[-1]) if p.returncode == 0 else {"error": p.returncode}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
     print(json.dumps(out))

@@ -18,6 +18,9 @@ The signals and gradients are means over the launched photons, so the
 backward pass is exact for the sample: grad_mua = g @ dS/dmua.
 """
 import os
+import pickle
+import select
+import struct
 import subprocess
 import sys
 
@@ -40,6 +43,7 @@ class Simulator:
     "slab" of lightpropagation_amd.scenes (built in the child, which holds
     the library).  det_lo/det_hi: annular detectors (exit radius in the
     emitter frame).  jac_grid: (origin, size, n) of the voxel mua Jacobian.
+    timeout: seconds to wait for any one reply before WorkerError.
     """
 
     def __init__(self, scene, seed, det_lo, det_hi, jac_grid=None, device=0, options=None, timeout=600):
@@ -52,7 +56,7 @@ class Simulator:
         os.close(r1)
         os.close(w2)
         self._out = os.fdopen(w1, "wb")
-        self._in = os.fdopen(r2, "rb")
+        self._in = r2  # raw descriptor: replies are read with a deadline
         self.timeout = timeout
         info = self.call("scene", spec=scene, seed=int(seed), det_lo=np.asarray(det_lo, np.float64),
                          det_hi=np.asarray(det_hi, np.float64), jac_grid=jac_grid, device=device,
@@ -67,13 +71,31 @@ class Simulator:
             raise WorkerError("simulator is closed")
         try:
             _w.send(self._out, (cmd, kw))
-            status, val = _w.recv(self._in)
+            (n,) = struct.unpack("<Q", self._read(8))
+            status, val = pickle.loads(self._read(n))
         except (EOFError, BrokenPipeError) as e:
             rc = self.proc.poll()
             raise WorkerError("worker process ended (exit status %s) during %r" % (rc, cmd)) from e
         if status != "ok":
             raise WorkerError(val)
         return val
+
+    def _read(self, n):
+        """n bytes of the reply, or WorkerError once `timeout` passes
+        without any."""
+        buf = bytearray()
+        while len(buf) < n:
+            ready, _, _ = select.select([self._in], [], [], self.timeout)
+            if not ready:  # a late reply would answer the next request: end the worker
+                self.proc.kill()
+                self.proc.wait()
+                self.proc = None
+                raise WorkerError("no reply from the worker process in %g s" % self.timeout)
+            chunk = os.read(self._in, n - len(buf))
+            if not chunk:
+                raise EOFError("pipe closed")
+            buf += chunk
+        return bytes(buf)
 
     def optics(self, wavelength_nm, spo2_maternal, spo2_fetal):
         """(n, mua, mus, g) of the maternal-fetal tissues (lpm_mf_optics)."""
@@ -97,11 +119,14 @@ class Simulator:
             self.call("close")
         except WorkerError:
             pass
-        for f in (self._out, self._in):
-            try:
-                f.close()
-            except OSError:
-                pass
+        try:
+            self._out.close()
+        except OSError:
+            pass
+        try:
+            os.close(self._in)
+        except OSError:
+            pass
         try:
             self.proc.wait(timeout=60)
         except subprocess.TimeoutExpired:

@@ -42,6 +42,22 @@ def test_worker_protocol():
     assert p.wait(timeout=60) == 0
 
 
+def test_worker_reply_deadline():
+    # a worker that never answers: the read gives up after the timeout and
+    # ends the worker, so no late reply can answer a later request
+    r, w = os.pipe()
+    p = subprocess.Popen([sys.executable, "-c", "import time; time.sleep(60)"])
+    sim = object.__new__(TA.Simulator)
+    sim._in, sim._out, sim.timeout, sim.proc = r, os.fdopen(w, "wb"), 0.3, p
+    with pytest.raises(TA.WorkerError, match="no reply"):
+        sim._read(8)
+    assert p.poll() is not None and sim.proc is None
+    with pytest.raises(TA.WorkerError, match="closed"):
+        sim.call("ping")
+    sim._out.close()
+    os.close(r)
+
+
 class LinearSim:
     """S(mua, mus) = S0 + Ga (mua - mua0) + Gs (mus - mus0) on the float32
     values it receives, as the transport receives them."""

@@ -638,7 +638,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     __shared__ unsigned long long wcnt[kBlock / 64][kWc];
     float4* lnodes = smem;                         // 2 * nlds
     float4* lmat = smem + 2 * a.nlds;              // lmat_n: per instance, ntissue materials + (rsp, 0, 0, 0)
-    int32_t* lran3 = reinterpret_cast<int32_t*>(lmat + a.lmat_n);
+    float* lalb = reinterpret_cast<float*>(lmat + a.lmat_n);  // lmat_n: mua / (mua + mus) of each row
+    int32_t* lran3 = reinterpret_cast<int32_t*>(lalb + a.lmat_n);
     // adjoint: per-lane path length and interaction count per tissue, tissue-major
     float* adjL = reinterpret_cast<float*>(reinterpret_cast<char*>(lran3) + (RNG == 1 ? sizeof(Ran3Lds) : 0));
     uint32_t* adjN = reinterpret_cast<uint32_t*>(adjL + kAdjMaxTissues * kBlock);
@@ -657,6 +658,12 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 lmat[i] = make_float4(a.nmuasg[4 * i], a.nmuasg[4 * i + 1], a.nmuasg[4 * i + 2], a.nmuasg[4 * i + 3]);
             if (tid == 0) lmat[a.ntissue] = make_float4(a.rsp, 0.0f, 0.0f, 0.0f);
         }
+    }
+    __syncthreads();
+    // the absorbed fraction of each material row, the same division Absorb makes
+    for (int i = tid; i < a.ninst * (a.ntissue + 1); i += kBlock) {
+        const float4 m = lmat[i];
+        lalb[i] = (m.y > 0.0f && m.z > 0.0f) ? m.y / (m.y + m.z) : 0.0f;
     }
     __syncthreads();
     const int32_t ntp1 = a.ntissue + 1;  // material rows per instance
@@ -728,7 +735,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         // mus 0, which a photon never re-enters and where nothing was absorbed
         float absorbed = 0.0f;
         if (cm.y > 0.0f && cm.z > 0.0f) {
-            absorbed = ph.w * (cm.y / (cm.y + cm.z));
+            absorbed = ph.w * lalb[ph.inst * ntp1 + ph.curr];  // ph.w * (mua / (mua + mus))
             ph.w -= absorbed;
         }
         float xt = rng.next();
@@ -1360,7 +1367,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
 
 int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen)
 {
-    size_t lds = (size_t)(2 * a.nlds + a.lmat_n) * sizeof(float4);
+    size_t lds = (size_t)(2 * a.nlds + a.lmat_n) * sizeof(float4) + (size_t)a.lmat_n * sizeof(float);
     hipError_t e;
     const void* fn;
     if (a.rng_mode == 1) lds += sizeof(Ran3Lds);

@@ -178,19 +178,38 @@ Bvh build_bvh(const float* tri, const int32_t* med, int64_t ntri, int max_leaf)
         work.push_back(li);
     }
 
-    // Flatten: every tree node becomes a threaded BvhNode, numbered breadth
-    // first so the top levels are contiguous (and can be staged in LDS).
+    // Flatten: every tree node becomes a threaded BvhNode.  The first
+    // kBfsTop nodes are numbered breadth first, so the top levels are
+    // contiguous (and can be staged in LDS); each subtree below them follows
+    // depth first (pre-order), so a node's first child is the next node and
+    // a traversal walks forward through memory.  The order changes where
+    // nodes are stored, not the threaded visiting order.
+    const size_t kBfsTop = 2048;
     std::vector<int32_t> order;
     std::vector<int32_t> flat_of(nodes.size(), -1);
     std::deque<int32_t> q;
     if (ntri > 0) q.push_back(0);
-    while (!q.empty()) {
+    while (!q.empty() && order.size() < kBfsTop) {
         int32_t t = q.front();
         q.pop_front();
         flat_of[t] = (int32_t)order.size();
         order.push_back(t);
         if (nodes[t].count == 0)
             for (int c = 0; c < 2; ++c) q.push_back(nodes[t].child[c]);
+    }
+    std::vector<int32_t> stack;
+    for (int32_t r : q) {
+        stack.push_back(r);
+        while (!stack.empty()) {
+            int32_t t = stack.back();
+            stack.pop_back();
+            flat_of[t] = (int32_t)order.size();
+            order.push_back(t);
+            if (nodes[t].count == 0) {
+                stack.push_back(nodes[t].child[1]);
+                stack.push_back(nodes[t].child[0]);
+            }
+        }
     }
     std::vector<int32_t> miss_of(nodes.size(), -1);
     for (int32_t t : order) {  // parents precede children in BFS order

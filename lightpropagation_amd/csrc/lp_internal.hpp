@@ -35,7 +35,7 @@ struct TriRec {
 static_assert(sizeof(TriRec) == 48, "triangle layout");
 
 struct Bvh {
-    std::vector<BvhNode> nodes;  // breadth-first order (top levels first), root at 0
+    std::vector<BvhNode> nodes;  // first 2048 breadth first (top levels), then subtrees depth first; root at 0
     std::vector<TriRec> tris;    // leaf order
     int32_t depth = 0;
     float pad = 0.0f;

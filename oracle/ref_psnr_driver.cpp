@@ -1,4 +1,4 @@
-// Driver for the reference's own RNG sources (src/PSNR/PSNR.cpp and
+// Main program around the reference's own RNG sources (src/PSNR/PSNR.cpp and
 // src/PSNR/ran3.c), compiled where they lie by oracle/Makefile into
 // oracle/_ref/ref_psnr.  Test infrastructure only: it prints, as JSON, the
 // state the reference host hands to the kernel (PSNR::InitRand3,

@@ -3,8 +3,8 @@
  * transport of Minx0512/LightPropagation.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (lightpropagation_amd/)
- * may include, link or call this code.  It is used by tests/ as the checker,
- * by __graft_entry__.smoke() to check one small GPU invocation, and by
+ * may include, link or call this code.  tests/ use it to check the GPU
+ * results, __graft_entry__.smoke() to check one small GPU invocation, and
  * bench.py's cpu_baseline leg as the CPU restatement of the reference
  * algorithm (brute-force triangle search, one photon per OpenMP thread).
  *

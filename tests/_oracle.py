@@ -1,8 +1,8 @@
 """ctypes binding to the CPU oracle (oracle/liboracle.so).
 
 Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg load this module.  The oracle is the checker, never the
-product path.
+cpu_baseline leg load this module.  The oracle checks the GPU results; it is
+never the product path.
 """
 import ctypes as C
 import os

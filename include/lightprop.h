@@ -157,9 +157,13 @@ int lp_get_exit_records(lp_context *ctx, float *out, int64_t nphotons);
 /* Runs photons [first, first + count) with a per-step event trace (12 floats
  * per step: kind, face, tissue, w, p(3), v(3), step, draws) of up to
  * max_rec steps each; nrec[i] receives the number of records of photon i.
- * Tallies are accumulated as by lp_run. */
+ * Tallies are accumulated as by lp_run.  No reference counterpart: the
+ * step-by-step view the parity tests compare with the oracle's. */
 int lp_trace(lp_context *ctx, int64_t first, int32_t count, int32_t max_rec, float *out, int32_t *nrec);
 
+/* Kernel time and launch figures of the last lp_run / lp_adjoint /
+ * lp_run_batch (the reference's PrintTimeSpent, CalculatePropagation.cpp:
+ * 2063, prints wall time only). */
 int lp_get_stats(lp_context *ctx, lp_stats *out);
 
 /* ---- adjoint (no reference counterpart) ----
@@ -182,7 +186,9 @@ int lp_adjoint(lp_context *ctx, int64_t first, int64_t count, int64_t *signals, 
                int64_t *npairs);
 
 /* Evaluates the kernel's math core element-wise on the device (op 0 sqrt,
- * 1 divide, 2 log, 3 cos, 4 acos, 5 multiply, 6 add) over n (x, y) pairs. */
+ * 1 divide, 2 log, 3 cos, 4 acos, 5 multiply, 6 add) over n (x, y) pairs.
+ * No reference counterpart (the reference calls OpenCL built-ins): it lets
+ * the tests compare the device math with the oracle's bit for bit. */
 int lp_debug_math(lp_context *ctx, int32_t op, const float *xy, float *out, int64_t n);
 
 /* Runs a plain device-to-device copy of `bytes` (multiple of 16) through a
@@ -192,10 +198,12 @@ int lp_debug_copy(lp_context *ctx, int64_t bytes, double *ms);
 
 /* Launch shape override: persistent work-groups (0 = automatic) and BVH
  * nodes staged in LDS (-1 = automatic), safety grid cells per axis (0 =
- * off, -1 = automatic). */
+ * off, -1 = automatic).  Replaces the fixed 6 400 work-items per launch of
+ * the reference (CalculatePropagation.cpp:76). */
 int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32_t safety_cells);
 
-/* Tuning options (no effect on results, only on speed). */
+/* Tuning options (no effect on results, only on speed; no reference
+ * counterpart). */
 #define LP_OPT_GRID_BLOCKS 0    /* persistent work-groups, 0 = automatic            */
 #define LP_OPT_LDS_NODES 1      /* BVH nodes staged in LDS (0..2048)                */
 #define LP_OPT_SAFETY_CELLS 2   /* safety grid cells on the longest axis (0 = off)  */
@@ -239,7 +247,9 @@ int lp_set_option(lp_context *ctx, int32_t key, double value);
 int lp_run_batch(lp_context *ctx, const float *nmuasg, int32_t ninst, int64_t first, int64_t count, int64_t *det,
                  int64_t *region);
 
-/* Launch-option autotuner: coordinate search over waves per SIMD,
+/* Launch-option autotuner (no reference counterpart: the reference fixes
+ * its launch at 6 400 work-items, CalculatePropagation.cpp:76): coordinate
+ * search over waves per SIMD,
  * work-groups per CU, free iterations, refill threshold and the two search
  * batch thresholds, timing tally-free launches of `sample` photons (a
  * setting must win by 1 %), for up to `passes` passes; leaves the fastest

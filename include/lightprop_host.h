@@ -26,7 +26,8 @@ extern "C" {
 
 typedef struct lph_scene lph_scene;
 
-/* Text of the last failed lph_* call on this thread (never NULL). */
+/* Text of the last failed lph_* call on this thread (never NULL); the
+ * reference reports errors on std::cout and through its return codes. */
 const char *lph_last_error(void);
 
 /* Reads a configuration XML (ReadXML::ParseFile, src/IO/ReadXML.cpp:93-168
@@ -51,11 +52,13 @@ int lph_scene_materials(const lph_scene *s, float *nmuasg, int32_t *fetal_id);
  * (POI, eX, eY, eZ) = poi_basis[12]. */
 int lph_scene_source(const lph_scene *s, float pstart[3], float *vstart, int32_t *vocc, float poi_basis[12]);
 /* model extent, detection grid (x, y, z, r, phi, rho), PixelPerUnit factor,
- * roulette chance, weight threshold, number of photons. */
+ * roulette chance, weight threshold, number of photons (the values
+ * main.cpp:101-320 reads from the configuration through ReadXML). */
 int lph_scene_params(const lph_scene *s, float model_dims[3], float det_grid[6], float *factor, float *chance,
                      float *w_th, int64_t *nphotons);
 /* Sets mesh, materials, source, grid and (chance, w_th, factor, seed,
- * rng_mode, max_steps) on a transport context. */
+ * rng_mode, max_steps) on a transport context: the setter sequence of
+ * main.cpp:174-222 in one call. */
 int lph_scene_apply(const lph_scene *s, lp_context *ctx, uint64_t seed, int32_t rng_mode, int64_t max_steps);
 
 /* Emitter model (Emitter::CalculateEmitterModel, Emitter.cpp:186-339):

@@ -18,7 +18,8 @@
 extern "C" {
 #endif
 
-/* Maternal-fetal abdomen: concentric maternal spheres (skin, fat, uterine
+/* Maternal-fetal abdomen (synthetic benchmark scene, no reference
+ * counterpart): concentric maternal spheres (skin, fat, uterine
  * wall, then the amniotic cavity) tangent to z = 0 at the origin, centred at
  * (0, 0, r_maternal), over a fetal head of three concentric spheres (scalp,
  * skull, brain) whose top lies at depth `fetal_depth` on the z axis.

@@ -104,6 +104,20 @@ const char *lp_last_error(const lp_context *ctx);
  * BVH and the safety grid on the host and uploads them. */
 int lp_set_mesh(lp_context *ctx, const float *tri, const float *nrm, const int32_t *med, int64_t ntri);
 
+/* Several geometries in one mesh (no reference counterpart: the reference
+ * kernel searches one triangle list, MCLightPropagation.cl:1262-1270, and
+ * a simulation loads one mesh, CalculatePropagation.cpp:541-667): the mesh
+ * of lp_set_mesh is ngeom geometries, geometry g being the next
+ * tri_count[g] triangles (the counts add up to the mesh's).  Each geometry
+ * gets a BVH and near-face grid of its own, so a photon of geometry g meets
+ * only g's faces and takes exactly the path it takes in a mesh of g alone;
+ * geometries may overlap in space and share the tissue ids, source and
+ * detection grid.  lp_run_batch_geom picks each instance's geometry; lp_run,
+ * lp_trace, lp_adjoint and lp_autotune transport in geometry 0.  lp_set_mesh
+ * resets to one geometry.  The safety grid (lp_set_launch) needs a single
+ * geometry.  At most 64 geometries. */
+int lp_set_geometries(lp_context *ctx, const int64_t *tri_count, int32_t ngeom);
+
 /* nmuasg[4i..4i+3] = (n, mua, mus, g) of tissue i; tissue 0 is where photons
  * start.  Replaces SetTissueProperties (cpp:427-451) + SetFetalTissueID. */
 int lp_set_materials(lp_context *ctx, const float *nmuasg, int32_t ntissue, int32_t fetal_id);
@@ -246,6 +260,13 @@ int lp_set_option(lp_context *ctx, int32_t key, double value);
  * new optics only refill the device table.  ninst * (ntissue + 1) <= 1152. */
 int lp_run_batch(lp_context *ctx, const float *nmuasg, int32_t ninst, int64_t first, int64_t count, int64_t *det,
                  int64_t *region);
+
+/* lp_run_batch over uneven geometries in the same launch (no reference
+ * counterpart): instance i transports in geometry geom[i] of
+ * lp_set_geometries (geom NULL: geometry 0), so its tallies equal those of
+ * lp_run_batch of instance i on a mesh of that geometry alone, bit for bit. */
+int lp_run_batch_geom(lp_context *ctx, const float *nmuasg, int32_t ninst, const int32_t *geom, int64_t first,
+                      int64_t count, int64_t *det, int64_t *region);
 
 /* Launch-option autotuner (no reference counterpart: the reference fixes
  * its launch at 6 400 work-items, CalculatePropagation.cpp:76): coordinate

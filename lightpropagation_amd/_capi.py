@@ -76,6 +76,8 @@ _SIGS = {
     "lp_set_jacobian_grid": (C.c_int, [C.c_void_p, _fp, _fp, _ip]),
     "lp_adjoint": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _lp, _lp, _lp, _lp]),
     "lp_run_batch": (C.c_int, [C.c_void_p, _fp, C.c_int32, C.c_int64, C.c_int64, _lp, _lp]),
+    "lp_set_geometries": (C.c_int, [C.c_void_p, _lp, C.c_int32]),
+    "lp_run_batch_geom": (C.c_int, [C.c_void_p, _fp, C.c_int32, _ip, C.c_int64, C.c_int64, _lp, _lp]),
     "lp_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(TuneResult)]),
     "lp_debug_math": (C.c_int, [C.c_void_p, C.c_int32, _fp, _fp, C.c_int64]),
     "lp_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
@@ -155,6 +157,12 @@ class Context:
     def set_mesh(self, tri, nrm, med):
         tri, nrm, med = _f(tri).reshape(-1), _f(nrm).reshape(-1), _i(med).reshape(-1)
         self._chk(lib().lp_set_mesh(self._c, fp(tri), fp(nrm), ip(med), len(tri) // 9), "lp_set_mesh")
+
+    def set_geometries(self, tri_counts):
+        """lp_set_geometries: the mesh is consecutive geometries of these
+        triangle counts, each searched on its own."""
+        n = np.ascontiguousarray(tri_counts, np.int64)
+        self._chk(lib().lp_set_geometries(self._c, n.ctypes.data_as(_lp), len(n)), "lp_set_geometries")
 
     def set_materials(self, nmuasg, fetal_id):
         nm = _f(nmuasg).reshape(-1)
@@ -261,8 +269,9 @@ class Context:
                                    C.byref(pairs)), "lp_adjoint")
         return dict(signals=sig, grad=grad, jac=J, pairs=pairs.value)
 
-    def run_batch(self, nmuasg, first, count):
-        """lp_run_batch: nmuasg (ninst, ntissue, 4) -> dict(det (ninst, ndet,
+    def run_batch(self, nmuasg, first, count, geom=None):
+        """lp_run_batch (geom None) or lp_run_batch_geom (geom: geometry of
+        each instance): nmuasg (ninst, ntissue, 4) -> dict(det (ninst, ndet,
         2), region (ninst, ntissue)) as int64 fixed point (2^32 per unit)."""
         m = np.ascontiguousarray(nmuasg, np.float32)
         if m.ndim != 3 or m.shape[2] != 4:
@@ -271,9 +280,16 @@ class Context:
         nd = self._keep.get("ndet", 0)
         det = np.zeros((ni, max(nd, 0), 2), np.int64)
         reg = np.zeros((ni, nt), np.int64)
-        self._chk(lib().lp_run_batch(self._c, fp(m), ni, int(first), int(count),
-                                     det.ctypes.data_as(_lp) if nd else None, reg.ctypes.data_as(_lp)),
-                  "lp_run_batch")
+        dp = det.ctypes.data_as(_lp) if nd else None
+        if geom is None:
+            self._chk(lib().lp_run_batch(self._c, fp(m), ni, int(first), int(count), dp, reg.ctypes.data_as(_lp)),
+                      "lp_run_batch")
+        else:
+            g = np.ascontiguousarray(geom, np.int32)
+            if g.shape != (ni,):
+                raise ValueError("geom must hold one geometry per instance")
+            self._chk(lib().lp_run_batch_geom(self._c, fp(m), ni, ip(g), int(first), int(count), dp,
+                                              reg.ctypes.data_as(_lp)), "lp_run_batch_geom")
         return dict(det=det, region=reg)
 
     def autotune(self, sample=2_000_000, passes=2):

@@ -381,7 +381,41 @@ double point_tri_dist2_d(const double p[3], const double a[3], const double ab[3
 
 }  // namespace
 
-NearGrid build_near_grid(const Bvh& bvh, float r0, float cell)
+namespace {
+
+// Listing radius of a cell: r0 + the cell's half diagonal + 2 pads (a point
+// the kernel places in a cell lies within half_diag + 2 pad of its centre).
+double near_list_radius(float r0, float cell, float bvh_pad)
+{
+    return (double)r0 + 0.5 * std::sqrt(3.0) * (double)cell + 2.0 * (double)bvh_pad;
+}
+
+// Origin and cell counts of a grid reaching r_list + cell beyond [lo, hi].
+int64_t fit_frame(NearGrid& g, const float lo[3], const float hi[3], double r_list, float cell)
+{
+    int64_t ncell = 1;
+    const float grow = (float)(r_list + cell);
+    for (int a = 0; a < 3; ++a) {
+        g.origin[a] = lo[a] - grow;
+        g.n[a] = std::max<int32_t>(1, (int32_t)std::ceil(((double)hi[a] - lo[a] + 2.0 * grow) / cell) + 1);
+        g.inv_cell[a] = 1.0f / cell;
+        ncell *= g.n[a];
+    }
+    return ncell;
+}
+
+}  // namespace
+
+NearGrid near_grid_frame(const float lo[3], const float hi[3], float r0, float cell, float pad)
+{
+    NearGrid g;
+    if (!(r0 > 0) || !(cell > 0)) return g;
+    fit_frame(g, lo, hi, near_list_radius(r0, cell, pad), cell);
+    g.r0 = r0;
+    return g;
+}
+
+NearGrid build_near_grid(const Bvh& bvh, float r0, float cell, const NearGrid* frame)
 {
     NearGrid g;
     const int64_t nt = (int64_t)bvh.tris.size();
@@ -395,7 +429,7 @@ NearGrid build_near_grid(const Bvh& bvh, float r0, float cell)
     // touched cells.
     const double pad = 2.0 * (double)bvh.pad;
     const double half_diag = 0.5 * std::sqrt(3.0) * (double)cell;
-    const double r_list = (double)r0 + half_diag + pad;
+    const double r_list = near_list_radius(r0, cell, bvh.pad);
     const double r_touch = half_diag + pad;
     Box all;
     for (int64_t i = 0; i < nt; ++i) {
@@ -407,12 +441,22 @@ NearGrid build_near_grid(const Bvh& bvh, float r0, float cell)
         }
     }
     int64_t ncell = 1;
-    const float grow = (float)(r_list + cell);
-    for (int a = 0; a < 3; ++a) {
-        g.origin[a] = all.lo[a] - grow;
-        g.n[a] = std::max<int32_t>(1, (int32_t)std::ceil(((double)all.hi[a] - all.lo[a] + 2.0 * grow) / cell) + 1);
-        g.inv_cell[a] = 1.0f / cell;
-        ncell *= g.n[a];
+    if (frame) {
+        // the given frame must reach r_list + cell beyond the faces, as a
+        // fitted one does
+        const float grow = (float)(r_list + cell);
+        for (int a = 0; a < 3; ++a) {
+            g.origin[a] = frame->origin[a];
+            g.n[a] = frame->n[a];
+            g.inv_cell[a] = frame->inv_cell[a];
+            ncell *= g.n[a];
+            const double frame_hi = (double)g.origin[a] + (double)g.n[a] * cell;
+            if (!(g.origin[a] <= all.lo[a] - grow && frame_hi >= (double)all.hi[a] + grow &&
+                  g.inv_cell[a] == 1.0f / cell))
+                return NearGrid();
+        }
+    } else {
+        ncell = fit_frame(g, all.lo, all.hi, r_list, cell);
     }
     if (ncell > ((int64_t)1 << 31) - 1 || nt > ((int64_t)1 << 31) - 1) return NearGrid();
     g.r0 = r0;
@@ -544,6 +588,97 @@ NearGrid build_near_grid(const Bvh& bvh, float r0, float cell)
         std::memcpy(&g.cells[2 * c + 1], &f, sizeof f);
     }
     return g;
+}
+
+bool build_multi_mesh(const float* tri, const int32_t* med, const std::vector<int64_t>& first, int lds_per_part,
+                      float r0, float cell, MultiMesh& out, std::string& why)
+{
+    const int ng = (int)first.size() - 1;
+    out = MultiMesh();
+    std::vector<Bvh> part((size_t)ng);
+    Box all;
+    float pad = 0.0f;
+    for (int g = 0; g < ng; ++g) {
+        const int64_t f = first[g], n = first[g + 1] - first[g];
+        part[g] = build_bvh(tri + 9 * f, med + 2 * f, n, 4);
+        for (int64_t t = f; t < first[g + 1]; ++t)
+            for (int v = 0; v < 3; ++v) all.grow(tri + 9 * t + 3 * v);
+        pad = std::max(pad, part[g].pad);
+        out.bvh.depth = std::max(out.bvh.depth, part[g].depth);
+    }
+    out.bvh.pad = pad;
+    // node placement: the tops of all parts, then the rest of each part
+    std::vector<int32_t> top((size_t)ng), top_base((size_t)ng), rest_base((size_t)ng);
+    int64_t ntop = 0, nall = 0;
+    for (int g = 0; g < ng; ++g) {
+        const int64_t n = (int64_t)part[g].nodes.size();
+        top[g] = (int32_t)std::min<int64_t>(n, std::max(1, lds_per_part));
+        top_base[g] = (int32_t)ntop;
+        ntop += top[g];
+        nall += n;
+    }
+    if (nall > INT32_MAX || first[ng] >= (1 << 24)) {
+        why = "merged BVH exceeds its index fields";
+        return false;
+    }
+    int64_t rest = ntop;
+    for (int g = 0; g < ng; ++g) {
+        rest_base[g] = (int32_t)rest;
+        rest += (int64_t)part[g].nodes.size() - top[g];
+    }
+    out.bvh.nodes.resize((size_t)nall);
+    out.bvh.tris.resize((size_t)first[ng]);
+    out.root.resize((size_t)ng);
+    for (int g = 0; g < ng; ++g) {
+        auto place = [&](int32_t i) { return i < 0 ? i : i < top[g] ? top_base[g] + i : rest_base[g] + (i - top[g]); };
+        const Bvh& b = part[g];
+        for (size_t k = 0; k < b.nodes.size(); ++k) {
+            BvhNode n = b.nodes[k];
+            if (n.hit >= 0) {
+                n.hit = place(n.hit);
+            } else {  // leaf: its range moves to the part's place in the merged leaf order
+                const uint32_t lf = (uint32_t)n.hit & 0xffffffu, cnt = ((uint32_t)n.hit >> 24) & 0x7fu;
+                n.hit = (int32_t)(0x80000000u | (cnt << 24) | (uint32_t)(lf + first[g]));
+            }
+            n.miss = place(n.miss);
+            out.bvh.nodes[(size_t)place((int32_t)k)] = n;
+        }
+        for (size_t i = 0; i < b.tris.size(); ++i) {
+            TriRec r = b.tris[i];
+            r.id += (int32_t)first[g];
+            out.bvh.tris[(size_t)first[g] + i] = r;
+        }
+        out.root[g] = top_base[g];
+    }
+    // near grids in one frame, part after part
+    out.coff.assign((size_t)ng, 0u);
+    if (!(r0 > 0) || first[ng] == 0) return true;
+    const NearGrid frame = near_grid_frame(all.lo, all.hi, r0, cell, pad);
+    const int64_t ncell = (int64_t)frame.n[0] * frame.n[1] * frame.n[2];
+    if ((ncell + 1) * ng > (int64_t)UINT32_MAX) {
+        why = "merged near grid exceeds its index fields";
+        return false;
+    }
+    out.ng = frame;
+    for (int g = 0; g < ng; ++g) {
+        NearGrid q = build_near_grid(part[g], r0, cell, &frame);
+        if (q.cells.size() != (size_t)(2 * (ncell + 1))) {
+            why = "near grid of a geometry does not fit the shared frame";
+            return false;
+        }
+        const uint64_t base = out.ng.list.size();
+        if (base + q.list.size() > (uint64_t)UINT32_MAX) {
+            why = "merged near lists exceed 2^32 entries";
+            return false;
+        }
+        out.coff[g] = (uint32_t)(ncell + 1) * (uint32_t)g;
+        for (int64_t c = 0; c <= ncell; ++c) {
+            out.ng.cells.push_back(q.cells[2 * c] + (uint32_t)base);
+            out.ng.cells.push_back(q.cells[2 * c + 1]);
+        }
+        for (int32_t e : q.list) out.ng.list.push_back(e + (int32_t)first[g]);
+    }
+    return true;
 }
 
 }  // namespace lp

@@ -36,6 +36,8 @@ struct lp_context {
     std::vector<float> tri, nrm;
     std::vector<int32_t> med;
     int64_t ntri = -1;
+    // geometries: triangles [part_first[g], part_first[g + 1]) (empty: one)
+    std::vector<int64_t> part_first;
     std::vector<float> nmuasg;
     int32_t ntissue = 0, fetal_id = -1;
     float pstart[3] = {0, 0, 0};
@@ -74,6 +76,8 @@ struct lp_context {
     SafetyGrid sg;
     NearGrid ng;
     float near_r0 = 0.05f, near_cell = 0.0f;  // cell 0 = 2 r0
+    std::vector<int32_t> part_root;   // BVH root of each geometry
+    std::vector<uint32_t> part_coff;  // near-grid cell offset of each geometry
     DevBuf d_near_cells, d_near_tris;
     bool mesh_dirty = true, src_dirty = true, tallies_dirty = true;
 
@@ -233,13 +237,31 @@ int prepare(lp_context* c)
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     if (c->mesh_dirty) {
-        c->bvh = build_bvh(c->tri.data(), c->med.data(), c->ntri, 4);
         c->bvh_extent = 0.0f;
         for (float x : c->tri) c->bvh_extent = std::max(c->bvh_extent, std::fabs(x));
         int cells = c->safety_cells < 0 ? 0 : c->safety_cells;  // off unless asked: the free radius covers it
-        c->sg = cells > 0 ? build_safety_grid(c->bvh, c->tri.data(), c->ntri, cells) : SafetyGrid();
-        c->ng = c->near_r0 > 0 ? build_near_grid(c->bvh, c->near_r0, c->near_cell > 0 ? c->near_cell : 2 * c->near_r0)
-                               : NearGrid();
+        const float cell = c->near_cell > 0 ? c->near_cell : 2 * c->near_r0;
+        if (c->part_first.size() > 2) {
+            if (cells > 0) return fail(c, LP_EINVAL, "the safety grid needs a single geometry");
+            const int ng = (int)c->part_first.size() - 1;
+            const int lds = c->lds_override >= 0 ? c->lds_override : 512;
+            MultiMesh mm;
+            std::string why;
+            if (!build_multi_mesh(c->tri.data(), c->med.data(), c->part_first, std::max(1, lds / ng), c->near_r0, cell,
+                                  mm, why))
+                return fail(c, LP_EINVAL, "lp_set_geometries: %s", why.c_str());
+            c->bvh = std::move(mm.bvh);
+            c->ng = std::move(mm.ng);
+            c->part_root = std::move(mm.root);
+            c->part_coff = std::move(mm.coff);
+            c->sg = SafetyGrid();
+        } else {
+            c->bvh = build_bvh(c->tri.data(), c->med.data(), c->ntri, 4);
+            c->sg = cells > 0 ? build_safety_grid(c->bvh, c->tri.data(), c->ntri, cells) : SafetyGrid();
+            c->ng = c->near_r0 > 0 ? build_near_grid(c->bvh, c->near_r0, cell) : NearGrid();
+            c->part_root.assign(1, 0);
+            c->part_coff.assign(1, 0u);
+        }
         if (!c->ng.list.empty()) {
             if ((rc = upload(c, c->d_near_cells, c->ng.cells.data(), c->ng.cells.size() * sizeof(uint32_t)))) return rc;
             // the lists hold copies of the triangle records, so a query reads
@@ -368,6 +390,7 @@ void fill_args(lp_context* c, KArgs& a)
     }
     a.counter = (unsigned long long*)c->d_counter.p;
     a.ninst = 1;
+    a.ngeom = 1;
     a.lmat_n = kMaxTissues + 1;
 }
 
@@ -461,7 +484,28 @@ int lp_set_mesh(lp_context* c, const float* tri, const float* nrm, const int32_t
     for (int64_t i = 0; i < 2 * ntri; ++i)
         if (med[i] < 0 || med[i] > 0xffff) return fail(c, LP_EINVAL, "lp_set_mesh: tissue id out of range");
     c->ntri = ntri;
+    c->part_first.clear();
     c->mesh_dirty = true;
+    return LP_OK;
+}
+
+int lp_set_geometries(lp_context* c, const int64_t* tri_count, int32_t ngeom)
+{
+    if (!c) return LP_EINVAL;
+    if (c->ntri < 0) return fail(c, LP_ESTATE, "lp_set_geometries: no mesh (lp_set_mesh)");
+    if (!tri_count || ngeom < 1 || ngeom > kMaxGeoms)
+        return fail(c, LP_EINVAL, "lp_set_geometries: need 1 to %d geometries", kMaxGeoms);
+    std::vector<int64_t> f(1, 0);
+    for (int32_t g = 0; g < ngeom; ++g) {
+        if (tri_count[g] < 1) return fail(c, LP_EINVAL, "lp_set_geometries: geometry %d has no triangles", g);
+        f.push_back(f.back() + tri_count[g]);
+    }
+    if (f.back() != c->ntri)
+        return fail(c, LP_EINVAL, "lp_set_geometries: the counts add up to %lld of the mesh's %lld triangles",
+                    (long long)f.back(), (long long)c->ntri);
+    if (ngeom == 1) f.clear();
+    if (f != c->part_first) c->mesh_dirty = true;
+    c->part_first = std::move(f);
     return LP_OK;
 }
 
@@ -729,10 +773,23 @@ extern "C" {
 int lp_run_batch(lp_context* c, const float* nmuasg, int32_t ninst, int64_t first, int64_t count, int64_t* det,
                  int64_t* region)
 {
+    return lp_run_batch_geom(c, nmuasg, ninst, nullptr, first, count, det, region);
+}
+
+int lp_run_batch_geom(lp_context* c, const float* nmuasg, int32_t ninst, const int32_t* geom, int64_t first,
+                      int64_t count, int64_t* det, int64_t* region)
+{
     if (!c) return LP_EINVAL;
     if (!nmuasg || ninst < 1 || first < 0 || count < 0) return fail(c, LP_EINVAL, "lp_run_batch: bad arguments");
     int rc = prepare(c);
     if (rc) return rc;
+    const int32_t ngeom = (int32_t)c->part_root.size();
+    if (geom && ngeom > 1 && c->rng_mode != 0)
+        return fail(c, LP_EINVAL, "lp_run_batch: several geometries need the Philox streams (rng_mode 0)");
+    if (geom)
+        for (int32_t i = 0; i < ninst; ++i)
+            if (geom[i] < 0 || geom[i] >= ngeom)
+                return fail(c, LP_EINVAL, "lp_run_batch: instance %d: geometry %d outside [0, %d)", i, geom[i], ngeom);
     const int32_t nt = c->ntissue, nd = (int32_t)c->det_rlo.size();
     if ((int64_t)ninst * (nt + 1) > kMaxInstRows)
         return fail(c, LP_EINVAL, "lp_run_batch: %d instances of %d tissues exceed the %d-row material table", ninst,
@@ -746,7 +803,11 @@ int lp_run_batch(lp_context* c, const float* nmuasg, int32_t ninst, int64_t firs
             if (!(m[4 * t] > 0) || !(m[4 * t + 1] >= 0) || !(m[4 * t + 2] >= 0) || !(m[4 * t + 3] >= -1 && m[4 * t + 3] <= 1))
                 return fail(c, LP_EINVAL, "lp_run_batch: instance %d tissue %d: need n > 0, mua, mus >= 0, |g| <= 1", i, t);
         std::memcpy(&tab[(size_t)i * (nt + 1) * 4], m, (size_t)nt * 4 * sizeof(float));
-        tab[((size_t)i * (nt + 1) + nt) * 4] = rspecular(m, nt);
+        float* row = &tab[((size_t)i * (nt + 1) + nt) * 4];  // (rsp, root bits, cell offset bits, 0)
+        row[0] = rspecular(m, nt);
+        const int32_t g = geom ? geom[i] : 0;
+        std::memcpy(&row[1], &c->part_root[g], sizeof(int32_t));
+        std::memcpy(&row[2], &c->part_coff[g], sizeof(uint32_t));
     }
     if ((rc = dev_alloc(c, c->d_inst_mat, tab.size() * sizeof(float)))) return rc;
     if ((rc = dev_alloc(c, c->d_bdet, (size_t)std::max<int64_t>(ndw, 1) * sizeof(int64_t)))) return rc;
@@ -759,6 +820,7 @@ int lp_run_batch(lp_context* c, const float* nmuasg, int32_t ninst, int64_t firs
     a.region = (unsigned long long*)c->d_bregion.p;
     a.inst_mat = (const float*)c->d_inst_mat.p;
     a.ninst = ninst;
+    a.ngeom = geom ? ngeom : 1;
     a.lmat_n = std::max<int32_t>(kMaxTissues + 1, ninst * (nt + 1));
     a.inst_count = std::max<int64_t>(count, 1);
     a.first = first;

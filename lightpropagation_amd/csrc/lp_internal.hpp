@@ -3,12 +3,13 @@
 #include <cstring>
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 namespace lp {
 
-// Threaded (stackless) BVH node, 32 bytes.  Traversal starts at node 0 and
-// follows `hit` when the ray segment meets the (padded) box and `miss`
+// Threaded (stackless) BVH node, 32 bytes.  Traversal starts at the root
+// (node 0 for a single geometry; MultiMesh::root for several) and follows `hit` when the ray segment meets the (padded) box and `miss`
 // otherwise; miss == -1 ends the walk.  For a leaf, hit < 0 encodes the
 // triangle range: first = hit & 0xffffff, count = (hit >> 24) & 0x7f, and
 // after the leaf's triangles are tested the walk continues at `miss`.
@@ -88,7 +89,33 @@ struct NearGrid {
         return f;
     }
 };
-NearGrid build_near_grid(const Bvh& bvh, float r0, float cell);
+// A frame given, the grid takes its origin and cell counts (it must reach
+// r0 + cell beyond every face of bvh, as near_grid_frame makes it);
+// otherwise the grid's own frame is fitted to the faces.
+NearGrid build_near_grid(const Bvh& bvh, float r0, float cell, const NearGrid* frame = nullptr);
+// Frame of a near grid over faces inside the box [lo, hi] whose BVH pads are
+// at most pad: what build_near_grid fits for such faces.
+NearGrid near_grid_frame(const float lo[3], const float hi[3], float r0, float cell, float pad);
+
+// Several geometries in one set of arrays: part g is the triangles
+// [first[g], first[g + 1]) with a BVH and near-face grid of its own.  The
+// merged BVH holds the top nodes of every part first (each part's first
+// min(n_g, lds_per_part) nodes, breadth first, so one LDS stage covers the
+// top of every tree), then the rest of each part; root[g] is where part g's
+// walk starts.  Triangle ids and leaf ranges are global.  The parts' near
+// grids share one frame (fitted to all faces) and are stored part after
+// part, each with its sentinel: part g's cell c is cell coff[g] + c of the
+// merged grid, whose lists index the merged leaf order.
+struct MultiMesh {
+    Bvh bvh;
+    NearGrid ng;
+    std::vector<int32_t> root;
+    std::vector<uint32_t> coff;
+};
+// Returns false (with a reason in why) when the merged arrays would overflow
+// their index fields.
+bool build_multi_mesh(const float* tri, const int32_t* med, const std::vector<int64_t>& first, int lds_per_part,
+                      float r0, float cell, MultiMesh& out, std::string& why);
 
 constexpr int kMaxTissues = 64;
 constexpr int kMaxDetectors = 64;
@@ -163,12 +190,14 @@ struct KArgs {
     int32_t jac_n[3];
     // batched launch (inst_mat null = one instance with nmuasg and rsp):
     // ninst instances differing in their tissue optics, instance i's
-    // materials at inst_mat[i * (ntissue + 1) + t] and its (rsp, 0, 0, 0) at
-    // [i * (ntissue + 1) + ntissue]; photon slot k of [0, count) is index
-    // first + k % inst_count of instance k / inst_count; det and region
-    // tallies are per instance
+    // materials at inst_mat[i * (ntissue + 1) + t] and its (rsp, root, coff,
+    // 0) at [i * (ntissue + 1) + ntissue] (root, coff: int bits of the BVH
+    // root and near-grid cell offset of its geometry, read when ngeom > 1);
+    // photon slot k of [0, count) is index first + k % inst_count of
+    // instance k / inst_count; det and region tallies are per instance
     const float* inst_mat;  // 4 floats per row, 16-byte aligned
     int32_t ninst;
+    int32_t ngeom;  // > 1: instances select their geometry (root, coff); else every walk starts at node 0, cell 0
     int32_t lmat_n;  // float4 rows of the material table in LDS (>= ninst * (ntissue + 1))
     int64_t inst_count;
     // work
@@ -202,5 +231,7 @@ int launch_debug_copy(const void* src, void* dst, int64_t bytes, void* stream);
 int launch_zero_u64(void* p, int64_t n, void* stream);
 // rows of the per-instance material table a batched launch may stage in LDS
 constexpr int kMaxInstRows = 1152;
+// geometries one mesh may hold (lp_set_geometries)
+constexpr int kMaxGeoms = 64;
 
 }  // namespace lp

@@ -193,7 +193,7 @@ struct Hit {
 // three separate conditional updates (t, id, med), hipcc (ROCm 7.2, gfx950)
 // kept the previous med when a face was taken through the t == best branch
 // (a hit at exactly t = tmax), so the key form is also what keeps med right.
-__device__ __forceinline__ void find_face(const KArgs& a, const float4* lnodes, f3 p, f3 v, int32_t ref,
+__device__ __forceinline__ void find_face(const KArgs& a, const float4* lnodes, int32_t root, f3 p, f3 v, int32_t ref,
                                           float tmax, Hit& h, uint32_t& visits, uint32_t& tests)
 {
     h.t = tmax;
@@ -211,7 +211,7 @@ __device__ __forceinline__ void find_face(const KArgs& a, const float4* lnodes, 
     const float iy = (v.y != 0.0f) ? 1.0f / v.y : 3.0e38f;
     const float iz = (v.z != 0.0f) ? 1.0f / v.z : 3.0e38f;
     const float4* gn = reinterpret_cast<const float4*>(a.nodes);
-    int32_t n = 0;
+    int32_t n = root;
     while (n >= 0) {
         float4 lo, hi;
         if (n < a.nlds) {
@@ -320,12 +320,12 @@ __device__ __forceinline__ float point_tri_dist2(f3 p, f3 a, f3 ab, f3 ac)
 // Distance from p to the nearest face, searched out to `cap` (returns cap
 // when nothing is nearer).  Boxes are padded, so box distances are lower
 // bounds; the caller subtracts a rounding margin.
-__device__ __forceinline__ float mesh_distance(const KArgs& a, const float4* lnodes, f3 p, float cap,
+__device__ __forceinline__ float mesh_distance(const KArgs& a, const float4* lnodes, int32_t root, f3 p, float cap,
                                                uint32_t& visits)
 {
     float best2 = cap * cap;
     const float4* gn = reinterpret_cast<const float4*>(a.nodes);
-    int32_t n = 0;
+    int32_t n = root;
     while (n >= 0) {
         float4 lo, hi;
         if (n < a.nlds) {
@@ -359,7 +359,8 @@ __device__ __forceinline__ float mesh_distance(const KArgs& a, const float4* lno
 }
 
 // Cell of the near-face grid holding p, or -1 outside it (then no face lies
-// within near_r0 of p).
+// within near_r0 of p), in the frame all geometries share (callers add the
+// cell offset of the photon's geometry).
 __device__ __forceinline__ int64_t near_cell(const KArgs& a, f3 p)
 {
     float fx = (p.x - a.ng_origin[0]) * a.ng_inv[0];
@@ -369,6 +370,25 @@ __device__ __forceinline__ int64_t near_cell(const KArgs& a, f3 p)
     int32_t ix = (int32_t)fminf(fx, 2.0e9f), iy = (int32_t)fminf(fy, 2.0e9f), iz = (int32_t)fminf(fz, 2.0e9f);
     if (ix >= a.ng_n[0] || iy >= a.ng_n[1] || iz >= a.ng_n[2]) return -1;
     return ((int64_t)iz * a.ng_n[1] + iy) * a.ng_n[0] + ix;
+}
+
+// BVH root and near-grid cell offset of an instance's geometry: the int bits
+// in the y and z words of its (rsp, root, coff, 0) material row in a launch
+// over several geometries (MG), node 0 and cell offset 0 otherwise.
+struct Geom {
+    int32_t root;
+    uint32_t coff;
+};
+template <bool MG>
+__device__ __forceinline__ Geom geom_of(const KArgs& a, const float4* lmat, int32_t inst)
+{
+    Geom g{0, 0u};
+    if (MG) {
+        const float4 r = lmat[inst * (a.ntissue + 1) + a.ntissue];
+        g.root = __float_as_int(r.y);
+        g.coff = __float_as_uint(r.z);
+    }
+    return g;
 }
 
 // Near-face list query, wave-cooperative, for the lanes with `part` set: the
@@ -382,9 +402,9 @@ __device__ __forceinline__ int64_t near_cell(const KArgs& a, f3 p)
 // distances from p to the listed faces with the record of the smaller;
 // unlisted = a lower bound on the distance to every face not in the list.
 // Must be called by the whole wavefront.
-__device__ __forceinline__ void near_batch(const KArgs& a, bool part, f3 p, f3 v, int32_t ref, float tmax, bool want,
-                                        Hit& h, float& d1sq, float& d2sq, int32_t& i1, float& unlisted,
-                                        uint32_t& pairs)
+__device__ __forceinline__ void near_batch(const KArgs& a, bool part, uint32_t coff, f3 p, f3 v, int32_t ref,
+                                        float tmax, bool want, Hit& h, float& d1sq, float& d2sq, int32_t& i1,
+                                        float& unlisted, uint32_t& pairs)
 {
     const int lane = threadIdx.x & 63;
     h.t = tmax;
@@ -397,6 +417,7 @@ __device__ __forceinline__ void near_batch(const KArgs& a, bool part, f3 p, f3 v
     if (part) {
         int64_t c = near_cell(a, p);
         if (c >= 0) {
+            c += coff;
             const CellRec cr = a.near_cells[c];
             b = cr.first;
             n = a.near_cells[c + 1].first - b;
@@ -631,7 +652,9 @@ enum WaveCount {
 // the compiler's choice); a tuning knob, results do not depend on it.
 // ADJ: the adjoint replay -- photons come from the detection list and every
 // move adds to the path-length gradients instead of the tallies.
-template <int RNG, bool TRACE, int WAVES, bool ADJ>
+// MG: a batched launch over several geometries -- each photon's searches
+// start at its instance's BVH root and near-grid cell offset.
+template <int RNG, bool TRACE, int WAVES, bool ADJ, bool MG>
 __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
 {
     extern __shared__ float4 smem[];
@@ -1177,7 +1200,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     // lookup places in the cell
                     const int64_t c = near_cell(a, ph.p);
                     if (c >= 0) {
-                        const float cl = __uint_as_float(a.near_cells[c].clear_bits);
+                        const float cl = __uint_as_float(a.near_cells[c + geom_of<MG>(a, lmat, ph.inst).coff].clear_bits);
                         const float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
                                                 __builtin_fabsf(ph.p.z));
                         const float sc = cl - (a.safe_margin + 1e-5f * (cl + rad));
@@ -1221,8 +1244,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 const bool want = ph.ref < 0 && !ballistic;
                 Hit hl;
                 uint32_t pairs;
-                near_batch(a, part, ph.p, ph.v, ph.ref, ballistic ? a.near_r0 : stepsize, want, hl, d1sq, d2sq, i1,
-                           unlisted, pairs);
+                near_batch(a, part, part ? geom_of<MG>(a, lmat, ph.inst).coff : 0u, ph.p, ph.v, ph.ref,
+                           ballistic ? a.near_r0 : stepsize, want, hl, d1sq, d2sq, i1, unlisted, pairs);
                 if (lane == 0) atomicAdd(&wc[wLst], (unsigned long long)pairs);
                 if (part && ballistic) {
                     if (hl.id >= 0) {  // the nearest face ahead, within r0
@@ -1285,7 +1308,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 if (__any(dq)) {
                     if (dq) {
                         float cap = fminf(a.dist_cap, a.dist_cap_steps * stepsize);
-                        float d = mesh_distance(a, lnodes, ph.p, cap, ldvis);
+                        float d = mesh_distance(a, lnodes, geom_of<MG>(a, lmat, ph.inst).root, ph.p, cap, ldvis);
                         float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
                                           __builtin_fabsf(ph.p.z));
                         ph.safe = fmaxf(0.0f, d - (a.safe_margin + 1e-5f * (d + rad)));  // >= 0: the test squares it
@@ -1302,7 +1325,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 }
                 // nearest face within the pending step
                 if (search && !by_list) {
-                    find_face(a, lnodes, ph.p, ph.v, ph.ref, ballistic ? 10000.0f : stepsize, h, lvis, ltst);
+                    find_face(a, lnodes, geom_of<MG>(a, lmat, ph.inst).root, ph.p, ph.v, ph.ref,
+                              ballistic ? 10000.0f : stepsize, h, lvis, ltst);
                     bnd = h.id >= 0 || ballistic || stepsize >= 10000.0f;  // no face ahead at all: escape
                     if (h.id < 0) h.t = 10000.0f;
                     if (ballistic) stepsize = h.t;
@@ -1372,19 +1396,28 @@ int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errl
     const void* fn;
     if (a.rng_mode == 1) lds += sizeof(Ran3Lds);
     const int w = a.waves;
-#define LP_PICK(R, T, W, J) (const void*)transport_kernel<R, T, W, J>
-    if (a.adj_list) {
+#define LP_PICK(R, T, W, J, M) (const void*)transport_kernel<R, T, W, J, M>
+    if (a.ngeom > 1 && (a.adj_list || a.rng_mode == 1 || a.trace)) {
+        snprintf(err, errlen, "transport launch: several geometries need the Philox streams, no trace or replay");
+        return 1;
+    }
+    if (a.ngeom > 1) {
+        fn = w == 4   ? LP_PICK(0, false, 4, false, true)
+             : w == 5 ? LP_PICK(0, false, 5, false, true)
+             : w == 6 ? LP_PICK(0, false, 6, false, true)
+                      : LP_PICK(0, false, 1, false, true);
+    } else if (a.adj_list) {
         lds += (size_t)kAdjMaxTissues * kBlock * (sizeof(float) + sizeof(uint32_t));
-        fn = a.rng_mode == 1 ? LP_PICK(1, false, 1, true) : LP_PICK(0, false, 1, true);
+        fn = a.rng_mode == 1 ? LP_PICK(1, false, 1, true, false) : LP_PICK(0, false, 1, true, false);
     } else if (a.rng_mode == 1) {
-        fn = a.trace ? LP_PICK(1, true, 1, false) : LP_PICK(1, false, 1, false);
+        fn = a.trace ? LP_PICK(1, true, 1, false, false) : LP_PICK(1, false, 1, false, false);
     } else if (a.trace) {
-        fn = LP_PICK(0, true, 1, false);
+        fn = LP_PICK(0, true, 1, false, false);
     } else {
-        fn = w == 4   ? LP_PICK(0, false, 4, false)
-             : w == 5 ? LP_PICK(0, false, 5, false)
-             : w == 6 ? LP_PICK(0, false, 6, false)
-                      : LP_PICK(0, false, 1, false);
+        fn = w == 4   ? LP_PICK(0, false, 4, false, false)
+             : w == 5 ? LP_PICK(0, false, 5, false, false)
+             : w == 6 ? LP_PICK(0, false, 6, false, false)
+                      : LP_PICK(0, false, 1, false, false);
     }
 #undef LP_PICK
     e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

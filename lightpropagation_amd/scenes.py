@@ -73,6 +73,27 @@ def maternal_fetal(wavelength_nm=660.0, spo2_maternal=0.97, spo2_fetal=0.50, cha
                 w_th=w_th)
 
 
+def geometries(scenes):
+    """One scene holding several geometries (for lp_set_geometries): their
+    meshes concatenated in order, everything else taken from the first,
+    which the others must share (source, grid, tissue count, parameters).
+    Returns (scene, triangle count of each geometry)."""
+    base = scenes[0]
+    for sc in scenes[1:]:
+        for k in ("pstart", "vstart", "vocc", "poi_basis", "model_dims", "det_grid"):
+            if not np.array_equal(np.asarray(sc[k]), np.asarray(base[k])):
+                raise ValueError("geometries differ in %s" % k)
+        for k in ("fetal_id", "factor", "chance", "w_th"):
+            if sc.get(k) != base.get(k):
+                raise ValueError("geometries differ in %s" % k)
+        if np.asarray(sc["nmuasg"]).shape != np.asarray(base["nmuasg"]).shape:
+            raise ValueError("geometries differ in their tissue count")
+    out = dict(base)
+    for k in ("tri", "nrm", "med"):
+        out[k] = np.ascontiguousarray(np.concatenate([np.asarray(sc[k]) for sc in scenes]))
+    return out, [len(sc["tri"]) for sc in scenes]
+
+
 def slab(layers, half=5.0, chance=0.1, w_th=1e-4, det_grid=(0.05, 0.05, 0.01, 0.05, 1, 1), distance=0.002):
     """Layered slab in the reference convention; layers = [(thickness, n, mua, mus, g), ...]."""
     nl = len(layers)

@@ -84,3 +84,21 @@ def test_large_mesh_size():
     p.freq_maternal = p.freq_fetal = 118
     p.freq_torso = 40
     assert S.lib().lpm_mf_num_triangles(C.byref(p)) == 7 * 20 * 118 * 118 + 20 * 40 * 40 == 1_981_360
+
+
+def test_geometries_merge():
+    """scenes.geometries concatenates the meshes in order and refuses scenes
+    that differ in anything but the mesh."""
+    a = S.maternal_fetal(fetal_depth=2.5, freq_maternal=6, freq_fetal=6)
+    b = S.maternal_fetal(fetal_depth=3.5, freq_maternal=8, freq_fetal=8)
+    m, counts = S.geometries([a, b])
+    assert counts == [len(a["tri"]), len(b["tri"])] and counts[0] != counts[1]
+    assert np.array_equal(m["tri"][:counts[0]], a["tri"]) and np.array_equal(m["tri"][counts[0]:], b["tri"])
+    assert np.array_equal(m["med"][counts[0]:], b["med"]) and np.array_equal(m["nrm"][:counts[0]], a["nrm"])
+    assert np.array_equal(m["pstart"], a["pstart"])
+    with pytest.raises(ValueError):
+        S.geometries([a, S.slab(S.C0_LAYERS)])
+    c = dict(b)
+    c["chance"] = 0.2
+    with pytest.raises(ValueError):
+        S.geometries([a, c])

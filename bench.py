@@ -339,6 +339,9 @@ def main():
     ap.add_argument("--lds-nodes", type=int, default=-1)
     ap.add_argument("--safety-cells", type=int, default=-1)
     ap.add_argument("--opt", action="append", default=[], help="tuning option key=value (lp_set_option)")
+    ap.add_argument("--tune", type=int, default=0, metavar="PHOTONS",
+                    help="run the autotuner (trials of this many photons, with the step's tallies) before the "
+                         "warm-up, untimed")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint vs finite-difference measurement")
     ap.add_argument("--adjoint-photons", type=int, default=10_000_000)
     ap.add_argument("--c2", type=int, default=0, metavar="PHOTONS",
@@ -372,6 +375,13 @@ def main():
     # is not initialised on the GPU here: it ships its own HIP/HSA runtime, and
     # a second runtime on the device in one process is not supported.
     print("setup done: %d faces" % T, file=sys.stderr, flush=True)
+    tuned = None
+    if args.tune:
+        r = ctx.autotune(sample=args.tune, passes=2)
+        tuned = {"sample": args.tune, "trials": r.trials, "ms_before": r.ms_before, "ms_after": r.ms_after,
+                 "waves": r.waves, "blocks_per_cu": r.blocks_per_cu, "free_iters": r.free_iters,
+                 "refill_min": r.refill_min, "list_min": r.list_min, "search_min": r.search_min}
+        print("autotune: %s" % tuned, file=sys.stderr, flush=True)
 
     first = 0
     for _ in range(args.warmup):
@@ -421,6 +431,7 @@ def main():
         "photon_steps_per_s": steps / wall,
         "photons_per_min": nph / wall * 60,
         "kernel_ms_per_step": sum(kms) / len(kms),
+        "autotune": tuned,
         "steps_per_photon": steps / nph,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,

@@ -270,12 +270,15 @@ int lp_run_batch_geom(lp_context *ctx, const float *nmuasg, int32_t ninst, const
 
 /* Launch-option autotuner (no reference counterpart: the reference fixes
  * its launch at 6 400 work-items, CalculatePropagation.cpp:76): coordinate
- * search over waves per SIMD,
- * work-groups per CU, free iterations, refill threshold and the two search
- * batch thresholds, timing tally-free launches of `sample` photons (a
- * setting must win by 1 %), for up to `passes` passes; leaves the fastest
- * setting on the context.  The options change how faces are found, never
- * which, so results do not depend on the outcome. */
+ * search over waves per SIMD, work-groups per CU, free iterations, refill
+ * threshold and the two search batch thresholds, for up to `passes` passes,
+ * timing launches of `sample` photons that carry the context's tallies (into
+ * stand-in buffers of the same sizes: the context's tallies are untouched).
+ * A setting must win by 1 %, and the final choice must win by 1 % again when
+ * it and the starting options are timed anew, alternately, twice each; else
+ * the starting options stay.  Leaves the chosen options on the context.  The
+ * options change how faces are found, never which, so results do not
+ * depend on the outcome.  Trial launches transport in geometry 0. */
 typedef struct lp_tune_result {
     int64_t sample_photons;
     int32_t trials;

@@ -6,7 +6,8 @@
 * The captured HIP graph, replayed with new optics between eager launches
   and host synchronisations, gives the eager batch's tallies every time.
 * The autotuner changes launch options only: the tallies after tuning equal
-  those before.
+  those before, and its trial launches (which carry the context's tallies
+  into stand-in buffers) leave the context's tallies untouched.
 """
 import numpy as np
 import pytest
@@ -79,10 +80,15 @@ def test_graph_replays_equal_eager():
 
 def test_autotune_keeps_results():
     ctx, sc = _ctx(seed=5)
+    ctx.set_voxel_grid([-6.4, -6.4, 0.0], [0.1, 0.1, 0.1], [128, 128, 128])
+    ctx.set_tallies(1 << lp.TALLY["det"] | 1 << lp.TALLY["region"] | 1 << lp.TALLY["vox"] | 1 << lp.TALLY["misc"])
     ctx.reset()
     ctx.run(0, 200_000)
     before = ctx.tallies()
     r = ctx.autotune(sample=400_000, passes=1)
+    during = ctx.tallies()  # the trial launches tally into stand-ins
+    for k in ("det", "region", "vox", "misc"):
+        assert before[k] == during[k] if k == "misc" else np.array_equal(before[k], during[k]), k
     print("autotune: %d trials, %.1f -> %.1f ms; waves %d, blocks/CU %d, free %d, refill %d, list %.3g, search %.3g"
           % (r.trials, r.ms_before, r.ms_after, r.waves, r.blocks_per_cu, r.free_iters, r.refill_min, r.list_min,
              r.search_min), flush=True)
@@ -90,8 +96,9 @@ def test_autotune_keeps_results():
     ctx.reset()
     ctx.run(0, 200_000)
     after = ctx.tallies()
-    for k in ("det", "region"):
+    for k in ("det", "region", "vox"):
         assert np.array_equal(before[k], after[k]), k
+    assert before["vox"].sum() > 0
     ctx.close()
 
 

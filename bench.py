@@ -88,9 +88,16 @@ def cpu_baseline(sc, seconds=20.0):
         done += batch
     dt = time.perf_counter() - t0
     steps = run.tallies()["misc"]["steps"]
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": done / dt, "unit": "photons/s", "cores": cores, "kind": "port",
             "sample": "%d photons of the same workload (%d steps) in %.1f s; photon-steps/s %.3g" %
-                      (done, steps, dt, steps / dt)}
+                      (done, steps, dt, steps / dt),
+            "host": {"cpu_model": model, "logical_cpus": os.cpu_count()}}
 
 
 def adjoint_vs_fd(sc, photons=10_000_000, batches=4, rels=(1e-3, 2e-2), jac_n=64):

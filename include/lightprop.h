@@ -223,7 +223,10 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
 #define LP_OPT_SAFETY_CELLS 2   /* safety grid cells on the longest axis (0 = off)  */
 #define LP_OPT_DIST_CAP 3       /* free-radius query: search cap in length units (0 = off) */
 #define LP_OPT_DIST_CAP_STEPS 4 /* free-radius query: cap in units of the step length */
-#define LP_OPT_BLOCKS_PER_CU 5  /* work-groups per CU when GRID_BLOCKS is automatic */
+#define LP_OPT_BLOCKS_PER_CU 5  /* work-groups per CU when GRID_BLOCKS is automatic (0,
+                                   default: as many as one CU holds at once, so the
+                                   persistent grid fills the chip at the variant's
+                                   occupancy)                                       */
 #define LP_OPT_FREE_ITERS 6     /* wavefront iterations per round (search batches are
                                    forced and finished lanes refilled at its end)  */
 #define LP_OPT_WAVES_PER_SIMD 7 /* register budget of the kernel variant: 1 (compiler's
@@ -241,6 +244,13 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
 #define LP_OPT_CLEAR_STEP 14    /* check a step that leaves its free region against the
                                    near-grid cell clearance when it is at least this
                                    long (0, default: every step; < 0: never)        */
+#define LP_OPT_BOUNDARY_RUN 15  /* end a photon as capped after this many boundary events
+                                   in a row, no interaction between (100 000, default;
+                                   0: no limit).  No reference counterpart: a photon
+                                   caught on the edge of two faces crosses from one to
+                                   the other at t ~ 0 for ever, as the reference's loop
+                                   would; this ends it (and changes a result) only
+                                   after that many crossings                        */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
 
 /* ---- batched launch (replaces ExecKernel's loop of 6 400-photon launches,
@@ -270,20 +280,24 @@ int lp_run_batch_geom(lp_context *ctx, const float *nmuasg, int32_t ninst, const
 
 /* Launch-option autotuner (no reference counterpart: the reference fixes
  * its launch at 6 400 work-items, CalculatePropagation.cpp:76): coordinate
- * search over waves per SIMD, work-groups per CU, free iterations, refill
- * threshold and the two search batch thresholds, for up to `passes` passes,
- * timing launches of `sample` photons that carry the context's tallies (into
- * stand-in buffers of the same sizes: the context's tallies are untouched).
- * A setting must win by 1 %, and the final choice must win by 1 % again when
- * it and the starting options are timed anew, alternately, twice each; else
- * the starting options stay.  Leaves the chosen options on the context.  The
- * options change how faces are found, never which, so results do not
- * depend on the outcome.  Trial launches transport in geometry 0. */
+ * search over waves per SIMD (the grid follows the variant's occupancy
+ * unless LP_OPT_BLOCKS_PER_CU is set), free iterations, refill threshold and
+ * the two search batch thresholds, for up to `passes` passes.
+ * A setting's score is the marginal time of `sample` photons: a launch of
+ * 2 sample photons less a launch of sample photons, so that the launch tail
+ * cancels and the choice suits long launches.  Trial launches carry the
+ * context's tallies (into stand-in buffers of the same sizes: the context's
+ * tallies are untouched) and transport in geometry 0.  A setting must win by
+ * 1 %, and the final choice must win by 1 % again when it and the starting
+ * options are scored anew, alternately, twice each; else the starting
+ * options stay.  Leaves the chosen options on the context.  The options
+ * change how faces are found, never which, so results do not depend on the
+ * outcome. */
 typedef struct lp_tune_result {
     int64_t sample_photons;
     int32_t trials;
-    double ms_before, ms_after; /* sample launch time with the initial and the chosen options */
-    int32_t waves, blocks_per_cu, free_iters, refill_min;
+    double ms_before, ms_after; /* score (marginal ms of sample photons) of the initial and the chosen options */
+    int32_t waves, blocks_per_cu, free_iters, refill_min; /* blocks_per_cu: in effect */
     double list_min, search_min;
 } lp_tune_result;
 int lp_autotune(lp_context *ctx, int64_t sample, int32_t passes, lp_tune_result *out);

@@ -50,6 +50,7 @@ struct lp_context {
     uint64_t seed = 1;
     int32_t rng_mode = 0;
     int64_t max_steps = 1000000;
+    int32_t max_bnd_run = 100000;
     float vox_origin[3] = {0, 0, 0}, vox_size[3] = {1, 1, 1};
     int32_t vox_n[3] = {0, 0, 0};
     float rad_dr = 0.05f;
@@ -59,7 +60,7 @@ struct lp_context {
     int32_t record_exit = 0;
     int32_t grid_override = 0, lds_override = -1, safety_cells = -1;
     float dist_cap = 0.2f, dist_cap_steps = 16.0f;
-    int32_t blocks_per_cu = 4;
+    int32_t blocks_per_cu = 0;  // 0: as many as one CU holds at once (the variant's occupancy)
     int32_t free_iters = 32;
     float search_min = 1.0f;
     float list_min = 0.25f;
@@ -365,6 +366,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.w_th = c->w_th;
     a.seed = c->seed;
     a.max_steps = c->max_steps;
+    a.max_bnd_run = (uint32_t)c->max_bnd_run;
     a.rng_mode = c->rng_mode;
     auto T = [&](int k) -> unsigned long long* {
         return (tally_on(c, k) && c->L.n[k] > 0) ? (unsigned long long*)c->d_tally[k].p : nullptr;
@@ -395,20 +397,26 @@ void fill_args(lp_context* c, KArgs& a)
     a.lmat_n = kMaxTissues + 1;
 }
 
-int grid_for(const lp_context* c, int64_t count)
+// Work-groups per CU of a launch with these arguments: the option, or the
+// number one CU holds at once, so that the persistent grid fills the chip
+// at the occupancy the kernel variant allows.
+int blocks_per_cu(const lp_context* c, const KArgs& a)
+{
+    return c->blocks_per_cu > 0 ? c->blocks_per_cu : transport_resident_blocks(a);
+}
+
+int grid_for(const lp_context* c, const KArgs& a)
 {
     int grid = c->grid_override;
-    if (grid <= 0) {
-        // enough work-groups to fill every CU at the occupancy the kernel allows
-        grid = std::max(1, c->num_cus) * c->blocks_per_cu;
-    }
+    if (grid <= 0) grid = std::max(1, c->num_cus) * blocks_per_cu(c, a);
+    const int64_t count = a.count;
     int64_t max_grid = (count + kBlock - 1) / kBlock;
     return (int)std::max<int64_t>(1, std::min<int64_t>(grid, std::max<int64_t>(max_grid, 1)));
 }
 
 int launch(lp_context* c, KArgs& a)
 {
-    const int grid = grid_for(c, a.count);
+    const int grid = grid_for(c, a);
     HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, 64, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     char err[256] = {0};
@@ -682,8 +690,12 @@ int lp_set_option(lp_context* c, int32_t key, double v)
         if (v < 1 || v > 64) return fail(c, LP_EINVAL, "refill lanes must be in [1, 64]");
         c->refill_min = (int32_t)v;
         return LP_OK;
+    case LP_OPT_BOUNDARY_RUN:
+        if (v < 0 || v > 1e9) return fail(c, LP_EINVAL, "boundary run must be in [0, 10^9] (0: no limit)");
+        c->max_bnd_run = (int32_t)v;
+        return LP_OK;
     case LP_OPT_BLOCKS_PER_CU:
-        if (v < 1 || v > 32) return fail(c, LP_EINVAL, "blocks per CU out of range");
+        if (v < 0 || v > 32) return fail(c, LP_EINVAL, "blocks per CU out of range (0: automatic)");
         c->blocks_per_cu = (int32_t)v;
         return LP_OK;
     default:
@@ -827,7 +839,7 @@ int lp_run_batch_geom(lp_context* c, const float* nmuasg, int32_t ninst, const i
     a.inst_count = std::max<int64_t>(count, 1);
     a.first = first;
     a.count = (int64_t)ninst * count;
-    const int grid = grid_for(c, a.count);
+    const int grid = grid_for(c, a);
     if (a.count == 0) {
         if (det) std::memset(det, 0, (size_t)ndw * sizeof(int64_t));
         if (region) std::memset(region, 0, (size_t)nrw * sizeof(int64_t));
@@ -874,7 +886,7 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
         if (k != LP_TALLY_MISC && on && c->L.n[k] > 0)
             if ((rc = dev_alloc(c, c->d_tune[k], (size_t)c->L.n[k] * sizeof(int64_t)))) return rc;
     }
-    auto trial = [&](double* ms) -> int {
+    auto launch_n = [&](int64_t n, double* ms) -> int {
         KArgs a;
         fill_args(c, a);
         auto stand_in = [&](unsigned long long* p, int k) {
@@ -890,9 +902,20 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
         a.det = stand_in(a.det, LP_TALLY_DET);
         a.misc = (unsigned long long*)c->d_adj_misc.p;
         a.first = 0;
-        a.count = sample;
+        a.count = n;
         int r = launch(c, a);
         *ms = c->stats.kernel_ms;
+        return r;
+    };
+    // A setting's score is the marginal time of `sample` photons -- a launch
+    // of 2 sample photons less one of sample photons -- so that the launch
+    // tail, which a short launch shortens by a smaller grid and a long one
+    // does not, cancels: the score is what a long launch pays per photon.
+    auto trial = [&](double* ms) -> int {
+        double t1 = 0, t2 = 0;
+        int r = launch_n(sample, &t1);
+        if (!r) r = launch_n(2 * sample, &t2);
+        *ms = t2 - t1;
         return r;
     };
     struct Knob {
@@ -904,7 +927,6 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
            sm = c->search_min;
     const double w0 = w, bpc0 = bpc, fi0 = fi, rm0 = rm, lm0 = lm, sm0 = sm;
     std::vector<Knob> knobs = {{"waves", {1, 4, 5, 6}, &w},
-                               {"blocks_per_cu", {2, 3, 4, 5, 6, 8}, &bpc},
                                {"free_iters", {16, 32, 64}, &fi},
                                {"refill_min", {8, 16, 24, 32}, &rm},
                                {"list_min", {0.125, 0.25, 0.5}, &lm},
@@ -984,7 +1006,9 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
         out->ms_before = ms0;
         out->ms_after = best;
         out->waves = c->waves;
-        out->blocks_per_cu = c->blocks_per_cu;
+        KArgs a;
+        fill_args(c, a);
+        out->blocks_per_cu = blocks_per_cu(c, a);  // in effect (the automatic value when the option is 0)
         out->free_iters = c->free_iters;
         out->refill_min = c->refill_min;
         out->list_min = c->list_min;

@@ -9,8 +9,9 @@
 namespace lp {
 
 // Threaded (stackless) BVH node, 32 bytes.  Traversal starts at the root
-// (node 0 for a single geometry; MultiMesh::root for several) and follows `hit` when the ray segment meets the (padded) box and `miss`
-// otherwise; miss == -1 ends the walk.  For a leaf, hit < 0 encodes the
+// (node 0 for a single geometry; MultiMesh::root for several) and follows
+// `hit` when the ray segment meets the (padded) box and `miss` otherwise;
+// miss == -1 ends the walk.  For a leaf, hit < 0 encodes the
 // triangle range: first = hit & 0xffffff, count = (hit >> 24) & 0x7f, and
 // after the leaf's triangles are tested the walk continues at `miss`.
 struct BvhNode {
@@ -162,6 +163,7 @@ struct KArgs {
     float factor, chance, w_th;
     uint64_t seed;
     int64_t max_steps;
+    uint32_t max_bnd_run;  // boundary events in a row that end a photon as capped (0: no limit)
     int32_t rng_mode;
     // tallies (null = off)
     unsigned long long *sft, *rtd, *arz, *phirz, *region, *vox, *rad, *det, *misc;
@@ -226,6 +228,9 @@ static_assert(kMiscUsed <= kMiscN, "misc tally too small for its counters");
 
 // Launchers (defined in lp_kernel.hip).
 int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen);
+// work-groups of the transport variant these arguments select that one CU
+// holds at once (registers and LDS), at least 1
+int transport_resident_blocks(const KArgs& a);
 int launch_debug_math(int op, const float* in, float* out, int64_t n, void* stream);
 int launch_debug_copy(const void* src, void* dst, int64_t bytes, void* stream);
 int launch_zero_u64(void* p, int64_t n, void* stream);

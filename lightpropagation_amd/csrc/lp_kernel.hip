@@ -557,6 +557,8 @@ struct Photon {
     f3 anc, pu;
     float safe, pc;
     int32_t ref, curr;
+    // fetal: bit 0 = traversed the fetal tissue; bits 1.. = boundary events
+    // since the last interaction
     int32_t fetal, roulette;
     int32_t inst;  // instance of a batched launch (0 otherwise)
     uint32_t steps;
@@ -920,7 +922,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 bool trans = dist > a.dist_min;
                 xsft.z = trans ? wr : 0.0f;
                 xsft.x = trans ? 0.0f : wr;
-                xsft.y = (!trans && ph.fetal) ? wr : 0.0f;
+                xsft.y = (!trans && (ph.fetal & 1)) ? wr : 0.0f;
                 ph.w = 0.0f;
             }
             if (new_id != ph.curr) {
@@ -929,7 +931,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             }
             ph.curr = new_id;
             ph.ref = h.id;
-            if (ph.curr == a.fetal_id) ph.fetal = 1;
+            if (ph.curr == a.fetal_id) ph.fetal |= 1;
             ph.unfin = (s - stb) * (cm.y + cm.z);
         }
         wcount(wBnd);
@@ -965,7 +967,13 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             t[10] = s;
             t[11] = (float)(TRACE ? rng.ndraw : 0);
         }
-        if (a.max_steps > 0 && (int64_t)ph.steps >= a.max_steps && ph.w > 0.0f) {
+        // step cap, and the boundary-run cap: a photon caught on the edge of
+        // two faces crosses from one to the other at t ~ 0 for ever (the
+        // last-face exclusion cannot separate them), which the reference
+        // loop never ends
+        ph.fetal = kind == 0 ? (ph.fetal & 1) : ph.fetal + 2;
+        if (ph.w > 0.0f && ((a.max_steps > 0 && (int64_t)ph.steps >= a.max_steps) ||
+                            (a.max_bnd_run > 0 && ((uint32_t)ph.fetal >> 1) >= a.max_bnd_run))) {
             wcount(wCap);
             ph.w = 0.0f;
         }
@@ -1389,12 +1397,13 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     }
 }
 
-int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen)
+namespace {
+
+// The kernel variant a launch with these arguments runs, and its dynamic LDS.
+int pick_transport(const KArgs& a, const void** fn, size_t* lds, char* err, int errlen)
 {
-    size_t lds = (size_t)(2 * a.nlds + a.lmat_n) * sizeof(float4) + (size_t)a.lmat_n * sizeof(float);
-    hipError_t e;
-    const void* fn;
-    if (a.rng_mode == 1) lds += sizeof(Ran3Lds);
+    *lds = (size_t)(2 * a.nlds + a.lmat_n) * sizeof(float4) + (size_t)a.lmat_n * sizeof(float);
+    if (a.rng_mode == 1) *lds += sizeof(Ran3Lds);
     const int w = a.waves;
 #define LP_PICK(R, T, W, J, M) (const void*)transport_kernel<R, T, W, J, M>
     if (a.ngeom > 1 && (a.adj_list || a.rng_mode == 1 || a.trace)) {
@@ -1402,25 +1411,35 @@ int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errl
         return 1;
     }
     if (a.ngeom > 1) {
-        fn = w == 4   ? LP_PICK(0, false, 4, false, true)
-             : w == 5 ? LP_PICK(0, false, 5, false, true)
-             : w == 6 ? LP_PICK(0, false, 6, false, true)
-                      : LP_PICK(0, false, 1, false, true);
+        *fn = w == 4   ? LP_PICK(0, false, 4, false, true)
+              : w == 5 ? LP_PICK(0, false, 5, false, true)
+              : w == 6 ? LP_PICK(0, false, 6, false, true)
+                       : LP_PICK(0, false, 1, false, true);
     } else if (a.adj_list) {
-        lds += (size_t)kAdjMaxTissues * kBlock * (sizeof(float) + sizeof(uint32_t));
-        fn = a.rng_mode == 1 ? LP_PICK(1, false, 1, true, false) : LP_PICK(0, false, 1, true, false);
+        *lds += (size_t)kAdjMaxTissues * kBlock * (sizeof(float) + sizeof(uint32_t));
+        *fn = a.rng_mode == 1 ? LP_PICK(1, false, 1, true, false) : LP_PICK(0, false, 1, true, false);
     } else if (a.rng_mode == 1) {
-        fn = a.trace ? LP_PICK(1, true, 1, false, false) : LP_PICK(1, false, 1, false, false);
+        *fn = a.trace ? LP_PICK(1, true, 1, false, false) : LP_PICK(1, false, 1, false, false);
     } else if (a.trace) {
-        fn = LP_PICK(0, true, 1, false, false);
+        *fn = LP_PICK(0, true, 1, false, false);
     } else {
-        fn = w == 4   ? LP_PICK(0, false, 4, false, false)
-             : w == 5 ? LP_PICK(0, false, 5, false, false)
-             : w == 6 ? LP_PICK(0, false, 6, false, false)
-                      : LP_PICK(0, false, 1, false, false);
+        *fn = w == 4   ? LP_PICK(0, false, 4, false, false)
+              : w == 5 ? LP_PICK(0, false, 5, false, false)
+              : w == 6 ? LP_PICK(0, false, 6, false, false)
+                       : LP_PICK(0, false, 1, false, false);
     }
 #undef LP_PICK
-    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return 0;
+}
+
+}  // namespace
+
+int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errlen)
+{
+    const void* fn = nullptr;
+    size_t lds = 0;
+    if (pick_transport(a, &fn, &lds, err, errlen)) return 1;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) {
         void* args[] = {const_cast<KArgs*>(&a)};
         e = hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, lds, (hipStream_t)stream);
@@ -1431,6 +1450,18 @@ int launch_transport(const KArgs& a, int grid, void* stream, char* err, int errl
         return 1;
     }
     return 0;
+}
+
+int transport_resident_blocks(const KArgs& a)
+{
+    const void* fn = nullptr;
+    size_t lds = 0;
+    char err[128];
+    if (pick_transport(a, &fn, &lds, err, sizeof err)) return 1;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return 1;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kBlock, lds) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
 }
 
 // ---------------------------------------------------------------------------

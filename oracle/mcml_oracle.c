@@ -687,6 +687,7 @@ static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc
     of3 nv = V(0.0f, 0.0f, -1.0f);
     int roulette = 1;
     int64_t steps = 0;
+    int64_t nbnd = 0; /* boundary events since the last interaction */
     int32_t ntr = 0;
     int64_t arz_out = 0, vox_out = 0;
 
@@ -803,7 +804,13 @@ static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc
             t[10] = stepsize; t[11] = (float)rng.ndraw;
             ntr++;
         }
-        if (P->max_steps > 0 && steps >= P->max_steps && w > 0.0f) {
+        /* Step cap, and the boundary-run cap: a photon caught on the edge of
+         * two faces reflects from one to the other at t ~ 0 for ever (the
+         * last-face exclusion cannot separate them; the reference loop never
+         * ends on it).  No reference counterpart. */
+        nbnd = kind == ORC_EV_INTERACT ? 0 : nbnd + 1;
+        if (w > 0.0f && ((P->max_steps > 0 && steps >= P->max_steps) ||
+                         (P->max_boundary_run > 0 && nbnd >= P->max_boundary_run))) {
             misc_loc[ORC_MISC_CAPPED]++;
             w = 0.0f;
         }

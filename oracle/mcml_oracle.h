@@ -88,6 +88,9 @@ typedef struct orc_params {
     int32_t rng_mode;
     uint64_t seed;
     int64_t max_steps; /* <= 0: unlimited */
+    /* boundary events in a row (no interaction between) after which a photon
+     * ends as capped, <= 0: unlimited (no reference counterpart) */
+    int64_t max_boundary_run;
     /* extra tallies (no reference counterpart) */
     float vox_origin[3];
     float vox_size[3];

@@ -30,7 +30,7 @@ class Params(C.Structure):
         ("pstart", C.c_float * 3), ("vstart", _fp), ("vocc", _ip), ("nstart", C.c_int32),
         ("poi_basis", C.c_float * 12), ("model_dims", C.c_float * 3), ("det_grid", C.c_float * 6),
         ("factor", C.c_float), ("chance", C.c_float), ("w_th", C.c_float),
-        ("rng_mode", C.c_int32), ("seed", C.c_uint64), ("max_steps", C.c_int64),
+        ("rng_mode", C.c_int32), ("seed", C.c_uint64), ("max_steps", C.c_int64), ("max_boundary_run", C.c_int64),
         ("vox_origin", C.c_float * 3), ("vox_size", C.c_float * 3), ("vox_n", C.c_int32 * 3),
         ("rad_dr", C.c_float), ("rad_n", C.c_int32),
         ("ndet", C.c_int32), ("det_rlo", _fp), ("det_rhi", _fp),
@@ -187,7 +187,7 @@ def vscatter(v, g, xt, xp):
 class Run:
     """Keeps the numpy buffers of one oracle call alive."""
 
-    def __init__(self, scene, rng_mode=RNG_PHILOX, seed=1, max_steps=0, tallies=None):
+    def __init__(self, scene, rng_mode=RNG_PHILOX, seed=1, max_steps=0, tallies=None, max_boundary_run=100_000):
         s = scene
         self.s = s
         self.tri = np.ascontiguousarray(s["tri"], np.float32).reshape(-1)
@@ -212,6 +212,7 @@ class Run:
         P.det_grid[:] = [float(x) for x in s["det_grid"]]
         P.factor, P.chance, P.w_th = s.get("factor", 1.0), s["chance"], s["w_th"]
         P.rng_mode, P.seed, P.max_steps = rng_mode, seed, max_steps
+        P.max_boundary_run = max_boundary_run  # the library's default
         vox = tl.get("vox")
         if vox:
             P.vox_origin[:] = list(vox["origin"])

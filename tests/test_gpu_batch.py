@@ -102,6 +102,22 @@ def test_autotune_keeps_results():
     ctx.close()
 
 
+def test_grid_follows_occupancy():
+    """The automatic grid holds as many work-groups per CU as one CU runs at
+    once: 4 for the 4-waves variant (128 VGPRs), 5 for the 5-waves one; an
+    explicit LP_OPT_BLOCKS_PER_CU overrides it."""
+    ctx, sc = _ctx()
+    grids = {}
+    for key, opts in (("w4", {}), ("w5", {"waves_per_simd": 5}), ("bpc2", {"waves_per_simd": 4, "blocks_per_cu": 2})):
+        for k, v in opts.items():
+            ctx.set_option(k, v)
+        ctx.run(0, 400_000)
+        grids[key] = ctx.stats().grid_blocks
+    print("grids", grids, flush=True)
+    ctx.close()
+    assert grids["w4"] % 4 == 0 and grids["w5"] * 4 == grids["w4"] * 5 and grids["bpc2"] * 2 == grids["w4"]
+
+
 def test_batch_rejects_oversized_tables():
     ctx, sc = _ctx()
     tab = np.repeat(_table([(660.0, 0.97, 0.5)]), 200, axis=0)  # 200 x 9 rows > 1152

@@ -338,7 +338,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--photons", type=int, default=16_000_000, help="photons per step")
+    ap.add_argument("--photons", type=int, default=100_000_000, help="photons per step (10^8: one wavelength of the C1 run)")
     ap.add_argument("--wavelength", type=float, default=660.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)

@@ -244,6 +244,10 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
 #define LP_OPT_CLEAR_STEP 14    /* check a step that leaves its free region against the
                                    near-grid cell clearance when it is at least this
                                    long (0, default: every step; < 0: never)        */
+#define LP_OPT_LIST_LONG 16     /* steps longer than the near-face radius try the lists
+                                   first (a face within the radius is the nearest)
+                                   before the BVH (1, default) or go to the BVH at
+                                   once (0)                                         */
 #define LP_OPT_BOUNDARY_RUN 15  /* end a photon as capped after this many boundary events
                                    in a row, no interaction between (100 000, default;
                                    0: no limit).  No reference counterpart: a photon

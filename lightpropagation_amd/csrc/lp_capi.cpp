@@ -66,6 +66,7 @@ struct lp_context {
     float list_min = 0.25f;
     int32_t refill_min = 16;
     float clear_step = 0.0f;
+    int32_t list_long = 1;
     int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
     float jac_origin[3] = {0, 0, 0}, jac_size[3] = {1, 1, 1};
@@ -326,6 +327,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.search_min = c->search_min;
     a.list_min = c->list_min;
     a.clear_step = c->clear_step;
+    a.list_long = c->list_long;
     a.refill_min = c->refill_min;
     a.waves = c->waves;
     a.safe_margin = 2e-6f * c->bvh_extent;
@@ -689,6 +691,10 @@ int lp_set_option(lp_context* c, int32_t key, double v)
     case LP_OPT_REFILL_MIN:
         if (v < 1 || v > 64) return fail(c, LP_EINVAL, "refill lanes must be in [1, 64]");
         c->refill_min = (int32_t)v;
+        return LP_OK;
+    case LP_OPT_LIST_LONG:
+        if (v != 0 && v != 1) return fail(c, LP_EINVAL, "list_long is 0 or 1");
+        c->list_long = (int32_t)v;
         return LP_OK;
     case LP_OPT_BOUNDARY_RUN:
         if (v < 0 || v > 1e9) return fail(c, LP_EINVAL, "boundary run must be in [0, 10^9] (0: no limit)");

@@ -1230,9 +1230,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     const bool finite_p =
                         __builtin_isfinite(ph.p.x) && __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
                     // the near-face lists answer a step of at most r0, and a
-                    // ballistic lane's flight when its face lies within r0
-                    // (a miss there falls back to the BVH)
-                    by_list = a.near_cells && finite_p && (ballistic || stepsize <= a.near_r0);
+                    // longer step or a ballistic lane's flight when its face
+                    // lies within r0 (a miss there falls back to the BVH)
+                    by_list = a.near_cells && finite_p && (ballistic || stepsize <= a.near_r0 || a.list_long);
                 }
             }
             const int nlive = __popcll(__ballot(alive));
@@ -1249,16 +1249,20 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 const bool part = search && by_list;
                 float d1sq, d2sq, unlisted;
                 int32_t i1;
-                const bool want = ph.ref < 0 && !ballistic;
+                // a flight or a step longer than r0: the lists see only its
+                // first r0 (every face nearer than r0 is listed, so a hit
+                // there is the nearest face of the whole mesh)
+                const bool reach = ballistic || stepsize > a.near_r0;
+                const bool want = ph.ref < 0 && !reach;
                 Hit hl;
                 uint32_t pairs;
                 near_batch(a, part, part ? geom_of<MG>(a, lmat, ph.inst).coff : 0u, ph.p, ph.v, ph.ref,
-                           ballistic ? a.near_r0 : stepsize, want, hl, d1sq, d2sq, i1, unlisted, pairs);
+                           reach ? a.near_r0 : stepsize, want, hl, d1sq, d2sq, i1, unlisted, pairs);
                 if (lane == 0) atomicAdd(&wc[wLst], (unsigned long long)pairs);
-                if (part && ballistic) {
+                if (part && reach) {
                     if (hl.id >= 0) {  // the nearest face ahead, within r0
                         h = hl;
-                        stepsize = h.t;
+                        if (ballistic) stepsize = h.t;
                         bnd = true;
                         search = false;
                         go = true;

@@ -192,7 +192,8 @@ def test_tuning_options_do_not_change_results():
     sc = S.maternal_fetal()
     settings = [{}, {"near_r0": 0.0}, {"near_r0": 0.03}, {"near_r0": 0.08, "near_cell": 0.1},
                 {"free_iters": 4, "search_min": 0.5}, {"waves_per_simd": 1, "lds_nodes": 0},
-                {"dist_cap": 0.0}, {"clear_step": -1.0, "refill_min": 4}, {"clear_step": 0.05}]
+                {"dist_cap": 0.0}, {"clear_step": -1.0, "refill_min": 4}, {"clear_step": 0.05},
+                {"list_long": 0}, {"list_long": 0, "near_r0": 0.03, "dist_cap": 0.0}]
     ref = None
     for opts in settings:
         ctx = lp.Context(0)

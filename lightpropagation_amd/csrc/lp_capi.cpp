@@ -402,7 +402,7 @@ void fill_args(lp_context* c, KArgs& a)
 // Work-groups per CU of a launch with these arguments: the option, or the
 // number one CU holds at once, so that the persistent grid fills the chip
 // at the occupancy the kernel variant allows.
-int blocks_per_cu(const lp_context* c, const KArgs& a)
+int effective_blocks_per_cu(const lp_context* c, const KArgs& a)
 {
     return c->blocks_per_cu > 0 ? c->blocks_per_cu : transport_resident_blocks(a);
 }
@@ -410,7 +410,7 @@ int blocks_per_cu(const lp_context* c, const KArgs& a)
 int grid_for(const lp_context* c, const KArgs& a)
 {
     int grid = c->grid_override;
-    if (grid <= 0) grid = std::max(1, c->num_cus) * blocks_per_cu(c, a);
+    if (grid <= 0) grid = std::max(1, c->num_cus) * effective_blocks_per_cu(c, a);
     const int64_t count = a.count;
     int64_t max_grid = (count + kBlock - 1) / kBlock;
     return (int)std::max<int64_t>(1, std::min<int64_t>(grid, std::max<int64_t>(max_grid, 1)));
@@ -929,9 +929,9 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
         std::vector<double> values;
         double* cur;
     };
-    double w = c->waves, bpc = c->blocks_per_cu, fi = c->free_iters, rm = c->refill_min, lm = c->list_min,
+    double w = c->waves, fi = c->free_iters, rm = c->refill_min, lm = c->list_min,
            sm = c->search_min;
-    const double w0 = w, bpc0 = bpc, fi0 = fi, rm0 = rm, lm0 = lm, sm0 = sm;
+    const double w0 = w, fi0 = fi, rm0 = rm, lm0 = lm, sm0 = sm;
     std::vector<Knob> knobs = {{"waves", {1, 4, 5, 6}, &w},
                                {"free_iters", {16, 32, 64}, &fi},
                                {"refill_min", {8, 16, 24, 32}, &rm},
@@ -939,7 +939,6 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
                                {"search_min", {0.5, 0.75, 1.0}, &sm}};
     auto apply = [&]() {
         c->waves = (int32_t)w;
-        c->blocks_per_cu = (int32_t)bpc;
         c->free_iters = (int32_t)fi;
         c->refill_min = (int32_t)rm;
         c->list_min = (float)lm;
@@ -979,15 +978,15 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
     }
     // A choice made on single timings must beat the starting options again
     // when both are timed anew, alternately, twice each; else they stay.
-    const double w1 = w, bpc1 = bpc, fi1 = fi, rm1 = rm, lm1 = lm, sm1 = sm;
+    const double w1 = w, fi1 = fi, rm1 = rm, lm1 = lm, sm1 = sm;
     if (best < ms0) {
         double t0 = 0, t1 = 0;
         for (int r = 0; r < 2; ++r) {
             for (int side = 0; side < 2; ++side) {
                 if (side == 0) {
-                    w = w0; bpc = bpc0; fi = fi0; rm = rm0; lm = lm0; sm = sm0;
+                    w = w0; fi = fi0; rm = rm0; lm = lm0; sm = sm0;
                 } else {
-                    w = w1; bpc = bpc1; fi = fi1; rm = rm1; lm = lm1; sm = sm1;
+                    w = w1; fi = fi1; rm = rm1; lm = lm1; sm = sm1;
                 }
                 apply();
                 double ms = 0;
@@ -999,10 +998,10 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
         ms0 = t0;
         best = t1;
         if (!(t1 < 0.99 * t0)) {
-            w = w0; bpc = bpc0; fi = fi0; rm = rm0; lm = lm0; sm = sm0;
+            w = w0; fi = fi0; rm = rm0; lm = lm0; sm = sm0;
             best = t0;
         } else {
-            w = w1; bpc = bpc1; fi = fi1; rm = rm1; lm = lm1; sm = sm1;
+            w = w1; fi = fi1; rm = rm1; lm = lm1; sm = sm1;
         }
         apply();
     }
@@ -1014,7 +1013,7 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
         out->waves = c->waves;
         KArgs a;
         fill_args(c, a);
-        out->blocks_per_cu = blocks_per_cu(c, a);  // in effect (the automatic value when the option is 0)
+        out->blocks_per_cu = effective_blocks_per_cu(c, a);  // in effect (the automatic value when the option is 0)
         out->free_iters = c->free_iters;
         out->refill_min = c->refill_min;
         out->list_min = c->list_min;

@@ -202,3 +202,8 @@ def slab(layers, half=5.0, ambient_n=1.0, distance=0.002, det_grid=(0.05, 0.05, 
 
 
 C0_LAYERS = [(0.15, 1.40, 0.4, 200.0, 0.9), (1.0, 1.44, 0.1, 120.0, 0.9), (2.0, 1.37, 0.3, 80.0, 0.9)]
+# ten layers (ten regions: 20 (mua, mus) parameters for the adjoint), 1 cm,
+# one index (under the signed-cosine Fresnel an inward crossing between
+# unequal indices almost always reflects, which would keep the photons in
+# the top layers)
+SLAB10_LAYERS = [(0.1, 1.40, 0.05 + 0.08 * i, 15.0 + 3.0 * i, 0.80 + 0.015 * i) for i in range(10)]

@@ -30,12 +30,15 @@ def _scene(name):
     if name == "c0slab":
         return scenes.slab(scenes.C0_LAYERS), [0.0, 0.3, 1.0], [0.3, 1.0, 3.0], \
             ([-5.0, -5.0, -0.01], [0.25, 0.25, 0.0205], [40, 40, 155])
+    if name == "slab10":  # ten regions: 20 (mua, mus) parameters
+        return scenes.slab(scenes.SLAB10_LAYERS), [0.0, 0.3, 1.0], [0.3, 1.0, 3.0], \
+            ([-5.0, -5.0, -0.01], [0.25, 0.25, 0.02], [40, 40, 52])
     from lightpropagation_amd import scenes as S
     return S.maternal_fetal(freq_maternal=6, freq_fetal=6), [0.0, 1.0, 3.0], [1.0, 3.0, 10.0], \
         ([-6.4, -6.4, 0.0], [0.2, 0.2, 0.2], [64, 64, 64])
 
 
-@pytest.mark.parametrize("name,n", [("c0slab", 20000), ("mf_coarse", 3000)])
+@pytest.mark.parametrize("name,n", [("c0slab", 20000), ("slab10", 20000), ("mf_coarse", 3000)])
 def test_adjoint_bit_exact_vs_oracle(name, n):
     sc, lo, hi, grid = _scene(name)
     print("adjoint", name, n, flush=True)

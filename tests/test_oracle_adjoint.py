@@ -64,8 +64,8 @@ def test_dda_matches_exact_overlaps():
         assert np.allclose(jac, want, rtol=1e-4, atol=2e-6 * max(float(s), 1.0)), trial
 
 
-def _c0_run(seed=3):
-    sc = scenes.slab(scenes.C0_LAYERS)
+def _c0_run(seed=3, layers=None):
+    sc = scenes.slab(layers or scenes.C0_LAYERS)
     tall = {"det_rlo": [0.0, 0.3, 1.0], "det_rhi": [0.3, 1.0, 3.0]}
     return sc, O.Run(sc, seed=seed, tallies=tall)
 
@@ -83,8 +83,10 @@ def test_voxel_and_tissue_gradients_agree():
         assert abs(voxels - tissue) <= 1e-5 * abs(tissue) + 1e-6, (d, voxels, tissue)
 
 
-def test_gradients_equal_estimator_on_traces():
-    sc, run = _c0_run(seed=11)
+@pytest.mark.parametrize("layers", ["c0", "slab10"])
+def test_gradients_equal_estimator_on_traces(layers):
+    # the three-layer slab, and ten layers: 10 regions x (mua, mus) = 20 parameters
+    sc, run = _c0_run(seed=11, layers=scenes.SLAB10_LAYERS if layers == "slab10" else None)
     N = 400
     sig, grad, _, pairs = run.adjoint(0, N)
     ex, tr, tn = run.run(0, N, exit_rec=True, trace_max=20000)
@@ -119,3 +121,5 @@ def test_gradients_equal_estimator_on_traces():
     # photon is never in a detected path
     assert np.allclose(grad / FX, want, rtol=2e-4, atol=1e-4)
     assert np.any(grad[:, 1:, 1] != 0)
+    if layers == "slab10":
+        assert np.all((grad[:, 1:, :] != 0).any(axis=0))  # every region reached by a detected path

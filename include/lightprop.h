@@ -210,6 +210,22 @@ int lp_debug_math(lp_context *ctx, int32_t op, const float *xy, float *out, int6
  * profiler's FETCH_SIZE / WRITE_SIZE counters into bytes. */
 int lp_debug_copy(lp_context *ctx, int64_t bytes, double *ms);
 
+/* Host only, no device: builds the merged arrays of a mesh of ngeom
+ * geometries as lp_set_geometries does (tri[9 * ntri], med[2 * ntri],
+ * tri_count[ngeom]; BVH tops of lds_per_part nodes each; near grid of radius
+ * r0 and cell size `cell`, r0 <= 0: none) and returns them for inspection.
+ * sizes[6] = (nodes, triangles, cell records incl. one sentinel per
+ * geometry, list entries, cells per geometry incl. its sentinel, 0); each
+ * output pointer may be NULL, else receives its whole array: nodes[8 * nodes]
+ * (lo.xyz, hit, hi.xyz, miss as raw 32-bit words), tri_ids[triangles] (the
+ * original id of each leaf-order triangle), roots[ngeom], coffs[ngeom],
+ * cells[2 * cell records] (first list entry, clearance bits), list[list
+ * entries] (leaf-order triangle of each entry).  No reference counterpart:
+ * the tests check the merged layout on the CPU. */
+int lp_debug_multi_mesh(const float *tri, const int32_t *med, const int64_t *tri_count, int32_t ngeom,
+                        int32_t lds_per_part, float r0, float cell, int64_t sizes[6], uint32_t *nodes,
+                        int32_t *tri_ids, int32_t *roots, uint32_t *coffs, uint32_t *cells, int32_t *list);
+
 /* Launch shape override: persistent work-groups (0 = automatic) and BVH
  * nodes staged in LDS (-1 = automatic), safety grid cells per axis (0 =
  * off, -1 = automatic).  Replaces the fixed 6 400 work-items per launch of

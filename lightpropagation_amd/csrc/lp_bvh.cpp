@@ -682,3 +682,33 @@ bool build_multi_mesh(const float* tri, const int32_t* med, const std::vector<in
 }
 
 }  // namespace lp
+
+extern "C" int lp_debug_multi_mesh(const float* tri, const int32_t* med, const int64_t* tri_count, int32_t ngeom,
+                                   int32_t lds_per_part, float r0, float cell, int64_t sizes[6], uint32_t* nodes,
+                                   int32_t* tri_ids, int32_t* roots, uint32_t* coffs, uint32_t* cells, int32_t* list)
+{
+    if (!tri || !med || !tri_count || ngeom < 1 || ngeom > lp::kMaxGeoms || !sizes) return 1;
+    std::vector<int64_t> first(1, 0);
+    for (int32_t g = 0; g < ngeom; ++g) {
+        if (tri_count[g] < 1) return 1;
+        first.push_back(first.back() + tri_count[g]);
+    }
+    lp::MultiMesh mm;
+    std::string why;
+    if (!lp::build_multi_mesh(tri, med, first, lds_per_part, r0, cell, mm, why)) return 2;
+    const int64_t ncell = (int64_t)mm.ng.n[0] * mm.ng.n[1] * mm.ng.n[2];
+    sizes[0] = (int64_t)mm.bvh.nodes.size();
+    sizes[1] = (int64_t)mm.bvh.tris.size();
+    sizes[2] = (int64_t)mm.ng.cells.size() / 2;
+    sizes[3] = (int64_t)mm.ng.list.size();
+    sizes[4] = mm.ng.cells.empty() ? 0 : ncell + 1;
+    sizes[5] = 0;
+    if (nodes && !mm.bvh.nodes.empty()) std::memcpy(nodes, mm.bvh.nodes.data(), mm.bvh.nodes.size() * sizeof(lp::BvhNode));
+    if (tri_ids)
+        for (size_t i = 0; i < mm.bvh.tris.size(); ++i) tri_ids[i] = mm.bvh.tris[i].id;
+    if (roots) std::memcpy(roots, mm.root.data(), mm.root.size() * sizeof(int32_t));
+    if (coffs) std::memcpy(coffs, mm.coff.data(), mm.coff.size() * sizeof(uint32_t));
+    if (cells && !mm.ng.cells.empty()) std::memcpy(cells, mm.ng.cells.data(), mm.ng.cells.size() * sizeof(uint32_t));
+    if (list && !mm.ng.list.empty()) std::memcpy(list, mm.ng.list.data(), mm.ng.list.size() * sizeof(int32_t));
+    return 0;
+}

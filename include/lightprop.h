@@ -271,6 +271,11 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
                                    the other at t ~ 0 for ever, as the reference's loop
                                    would; this ends it (and changes a result) only
                                    after that many crossings                        */
+#define LP_OPT_NO_SKIP 17      /* 1: search every step for its face (no free region, cell
+                                   clearance, safety grid or distance query); the
+                                   lists and the BVH still answer the searches.
+                                   For tests: the exhaustive form of the face
+                                   search, whose results the shortcuts must equal */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
 
 /* ---- batched launch (replaces ExecKernel's loop of 6 400-photon launches,

@@ -67,6 +67,7 @@ struct lp_context {
     int32_t refill_min = 16;
     float clear_step = 0.0f;
     int32_t list_long = 1;
+    int32_t no_skip = 0;
     int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
     float jac_origin[3] = {0, 0, 0}, jac_size[3] = {1, 1, 1};
@@ -328,6 +329,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.list_min = c->list_min;
     a.clear_step = c->clear_step;
     a.list_long = c->list_long;
+    a.no_skip = c->no_skip;
     a.refill_min = c->refill_min;
     a.waves = c->waves;
     a.safe_margin = 2e-6f * c->bvh_extent;
@@ -695,6 +697,10 @@ int lp_set_option(lp_context* c, int32_t key, double v)
     case LP_OPT_LIST_LONG:
         if (v != 0 && v != 1) return fail(c, LP_EINVAL, "list_long is 0 or 1");
         c->list_long = (int32_t)v;
+        return LP_OK;
+    case LP_OPT_NO_SKIP:
+        if (v != 0 && v != 1) return fail(c, LP_EINVAL, "no_skip is 0 or 1");
+        c->no_skip = (int32_t)v;
         return LP_OK;
     case LP_OPT_BOUNDARY_RUN:
         if (v < 0 || v > 1e9) return fail(c, LP_EINVAL, "boundary run must be in [0, 10^9] (0: no limit)");
@@ -1111,6 +1117,12 @@ int lp_adjoint(lp_context* c, int64_t first, int64_t count, int64_t* signals, in
     const int32_t nd = (int32_t)c->det_rlo.size(), nt = c->ntissue;
     if (nd < 1) return fail(c, LP_ESTATE, "lp_adjoint: no detectors (lp_set_detectors)");
     if (nt > kAdjMaxTissues) return fail(c, LP_EINVAL, "lp_adjoint: at most %d tissues", kAdjMaxTissues);
+    // The estimator differentiates the sampling density mu_t exp(-mu_t s) of
+    // a step length s = -ln(xi) / mu_t: with PixelPerUnit factor != 1 a
+    // fresh step is -ln(xi) factor / mu_t while a step carried over a
+    // boundary is not scaled, and that density no longer holds.
+    if (c->factor != 1.0f)
+        return fail(c, LP_EINVAL, "lp_adjoint: needs PixelPerUnit factor 1 (the path-length estimator assumes it)");
     const int64_t nvox = (int64_t)c->jac_n[0] * c->jac_n[1] * c->jac_n[2];
     // capacity: photons x the most detectors any exit radius can lie in
     int32_t overlap = 0;
@@ -1135,12 +1147,18 @@ int lp_adjoint(lp_context* c, int64_t first, int64_t count, int64_t* signals, in
     a.det_cap = cap;
     c->exit_count = 0;
     if (count > 0 && (rc = launch(c, a))) return rc;
-    const double fwd_ms = c->stats.kernel_ms;
+    const double fwd_ms = count > 0 ? c->stats.kernel_ms : 0.0;
+    // every copy below is ordered on the context's (non-blocking) stream and
+    // waited for, also when no launch ran to synchronise it
     unsigned long long n = 0;
-    HIPCHK(c, hipMemcpy(&n, c->d_det_count.p, 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(&n, c->d_det_count.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((int64_t)n > cap) return fail(c, LP_EDEVICE, "lp_adjoint: detection list overflow");
     std::vector<DetRec> list(n);
-    if (n) HIPCHK(c, hipMemcpy(list.data(), c->d_det_list.p, n * sizeof(DetRec), hipMemcpyDeviceToHost));
+    if (n) {
+        HIPCHK(c, hipMemcpyAsync(list.data(), c->d_det_list.p, n * sizeof(DetRec), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     if (signals) {
         std::fill(signals, signals + nd, 0);
         for (const DetRec& r : list) signals[r.det] += (int64_t)(r.w * 4294967296.0f);
@@ -1178,9 +1196,10 @@ int lp_adjoint(lp_context* c, int64_t first, int64_t count, int64_t* signals, in
         rep_ms = c->stats.kernel_ms;
     }
     int64_t misc[kMiscN];
-    HIPCHK(c, hipMemcpy(misc, c->d_adj_misc.p, sizeof misc, hipMemcpyDeviceToHost));
-    if (grad) HIPCHK(c, hipMemcpy(grad, c->d_grad.p, gbytes, hipMemcpyDeviceToHost));
-    if (jbytes) HIPCHK(c, hipMemcpy(jac, c->d_jac.p, jbytes, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(misc, c->d_adj_misc.p, sizeof misc, hipMemcpyDeviceToHost, c->stream));
+    if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->d_grad.p, gbytes, hipMemcpyDeviceToHost, c->stream));
+    if (jbytes) HIPCHK(c, hipMemcpyAsync(jac, c->d_jac.p, jbytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (npairs) *npairs = (int64_t)n;
     c->stats.kernel_ms = fwd_ms;
     c->stats.photons = count;

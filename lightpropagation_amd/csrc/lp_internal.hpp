@@ -148,6 +148,7 @@ struct KArgs {
     float list_min;        // run a near-list search batch when this fraction of live lanes waits for one
     float clear_step;      // cell-clearance free check for steps of at least this length (< 0: off)
     int32_t list_long;     // steps longer than near_r0 try the near-face lists (a face within near_r0) before the BVH
+    int32_t no_skip;       // 1: search every step (no free region, clearance, safety grid or distance query)
     int32_t refill_min;    // end the round once this many lanes of a wavefront have finished
     int32_t waves;         // occupancy target of the kernel variant (1, 4, 5, 6)
     float nmuasg[4 * kMaxTissues];

@@ -1201,8 +1201,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 // free step: the end point stays inside the free region
                 const f3 q = add(ph.p, mul(stepsize, ph.v));
                 const f3 e = sub(q, ph.anc);
-                freed = !ballistic && dot(e, e) < ph.safe * ph.safe && dot(ph.pu, q) > ph.pc;
-                if (!freed && !ballistic && a.near_cells && a.clear_step >= 0.0f && stepsize >= a.clear_step) {
+                freed = !a.no_skip && !ballistic && dot(e, e) < ph.safe * ph.safe && dot(ph.pu, q) > ph.pc;
+                if (!freed && !a.no_skip && !ballistic && a.near_cells && a.clear_step >= 0.0f &&
+                    stepsize >= a.clear_step) {
                     // else the ball of the near-grid cell's clearance around p:
                     // no face lies within the clearance of any point the cell
                     // lookup places in the cell
@@ -1222,7 +1223,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                         }
                     }
                 }
-                if (freed || (!ballistic && a.safety && stepsize < safety_at(a, ph.p))) {
+                if (freed || (!a.no_skip && !ballistic && a.safety && stepsize < safety_at(a, ph.p))) {
                     go = true;
                     wcount(wSkip);
                 } else {
@@ -1253,7 +1254,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 // first r0 (every face nearer than r0 is listed, so a hit
                 // there is the nearest face of the whole mesh)
                 const bool reach = ballistic || stepsize > a.near_r0;
-                const bool want = ph.ref < 0 && !reach;
+                const bool want = !a.no_skip && ph.ref < 0 && !reach;
                 Hit hl;
                 uint32_t pairs;
                 near_batch(a, part, part ? geom_of<MG>(a, lmat, ph.inst).coff : 0u, ph.p, ph.v, ph.ref,
@@ -1315,7 +1316,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 const bool finite_p =
                     __builtin_isfinite(ph.p.x) && __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
                 // a distance query capped below the step cannot free it
-                bool dq = sb && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && finite_p &&
+                bool dq = sb && !a.no_skip && !ballistic && a.dist_cap > 0.0f && ph.ref < 0 && finite_p &&
                           stepsize < fminf(a.dist_cap, a.dist_cap_steps * stepsize);
                 if (__any(dq)) {
                     if (dq) {

@@ -645,6 +645,52 @@ static void dda_add(const orc_adjoint_out *A, int64_t *jac_d, const float *q, fl
     }
 }
 
+/* TransmitOrReflect (.cl:921-1018) with GetCosAlphaCrit (:909-919),
+ * GetFresnel (:857-907), Transmit (:506-529) and Reflect (:532-549): the
+ * direction after a boundary event at unit normal nv between index n1 (the
+ * photon's tissue) and n2, for the Fresnel draw x.  cos_in = nv . v is
+ * signed as in the reference.  Returns ORC_EV_TRANSMIT or ORC_EV_REFLECT.
+ * ORC_PLANT_* (test builds only) plant one mistake each, so the tests can
+ * show that their checks catch it. */
+static int transmit_or_reflect(of3 *vp, of3 nv, float n1, float n2, float x)
+{
+    const of3 v = *vp;
+    const float cos_in = vdot(nv, v);
+    const float abs_cos = fabsf(cos_in);
+#ifdef ORC_PLANT_SWAP_N
+    { float q = n1; n1 = n2; n2 = q; }
+#endif
+    float cos_crit = 0.0f;
+    if (n1 > n2) { float q = n2 / n1; cos_crit = sqrtf(1.0f - q * q); }
+    const float r = (abs_cos <= cos_crit) ? 1.0f : orc_fresnel(n1, n2, cos_in);
+    const of3 ey = vnormalize(vcross(nv, vcross(nv, v)));
+    if (x > r) { /* Transmit (.cl:506-529) */
+        float sin_t = sqrtf(orc_clamp0(1 - cos_in * cos_in)) * n1 / n2;
+#ifdef ORC_PLANT_FLIP_SIN_T
+        sin_t = -sin_t;
+#endif
+        float cos_t = -sqrtf(orc_clamp0(1 - sin_t * sin_t));
+        if (cos_in > 0.0f) cos_t = fabsf(cos_t);
+        *vp = vnormalize(vsub(vmul(cos_t, nv), vmul(sin_t, ey)));
+        return ORC_EV_TRANSMIT;
+    }
+    /* Reflect (.cl:532-549) */
+    float sin_r = sqrtf(orc_clamp0(1 - cos_in * cos_in));
+#ifdef ORC_PLANT_FLIP_EY
+    sin_r = -sin_r;
+#endif
+    *vp = vneg(vnormalize(vadd(vmul(cos_in, nv), vmul(sin_r, ey))));
+    return ORC_EV_REFLECT;
+}
+
+int orc_transmit_or_reflect(const float nv[3], float v[3], float n1, float n2, float x)
+{
+    of3 w = V(v[0], v[1], v[2]);
+    int kind = transmit_or_reflect(&w, V(nv[0], nv[1], nv[2]), n1, n2, x);
+    v[0] = w.x; v[1] = w.y; v[2] = w.z;
+    return kind;
+}
+
 /* ------------------------------------------------------------------ */
 /* One photon: kernel body .cl:1299-1598                                */
 /* ------------------------------------------------------------------ */
@@ -726,29 +772,16 @@ static void run_photon(const orc_params *P, const orc_layout *L, int64_t id, orc
                 if (fabsf(sqrtf(p.x * p.x + p.y * p.y)) < 0.01f)
                     nv = vmul(orc_sign(vdot(nv, eZ)), vnormalize(eZ));
                 /* TransmitOrReflect (.cl:921-1018) */
-                float cos_in = vdot(nv, v);
-                float abs_cos = fabsf(cos_in);
-                float n1 = c_n, n2 = n_n;
-                float cos_crit = 0.0f;
-                if (n1 > n2) { float q = n2 / n1; cos_crit = sqrtf(1.0f - q * q); }
-                float r = (abs_cos <= cos_crit) ? 1.0f : orc_fresnel(n1, n2, cos_in);
+                const float cos_in = vdot(nv, v);
                 int32_t new_id;
                 float m_a, m_s;
-                of3 ey = vnormalize(vcross(nv, vcross(nv, v)));
-                if (x > r) { /* Transmit (.cl:506-529) */
-                    float sin_t = sqrtf(orc_clamp0(1 - cos_in * cos_in)) * n1 / n2;
-                    float cos_t = -sqrtf(orc_clamp0(1 - sin_t * sin_t));
-                    if (cos_in > 0.0f) cos_t = fabsf(cos_t);
-                    v = vnormalize(vsub(vmul(cos_t, nv), vmul(sin_t, ey)));
+                kind = transmit_or_reflect(&v, nv, c_n, n_n, x);
+                if (kind == ORC_EV_TRANSMIT) {
                     new_id = (cos_in > 0) ? out_id : in_id;
                     m_a = n_a; m_s = n_s;
-                    kind = ORC_EV_TRANSMIT;
-                } else {    /* Reflect (.cl:532-549) */
-                    float sin_r = sqrtf(orc_clamp0(1 - cos_in * cos_in));
-                    v = vneg(vnormalize(vadd(vmul(cos_in, nv), vmul(sin_r, ey))));
+                } else {
                     new_id = curr;
                     m_a = c_a; m_s = c_s;
-                    kind = ORC_EV_REFLECT;
                 }
                 if (m_a == 0 || m_s == 0) { /* leaves the tissue: record and kill */
                     sft.y = 0;
@@ -884,6 +917,8 @@ int orc_adjoint(const orc_params *P, int64_t first, int64_t count, orc_tallies *
                 int nthreads)
 {
     if (!P || !out || count < 0 || P->ntissue < 2 || P->nstart < 1) return 1;
+    /* the path-length estimator assumes step lengths -ln(xi) / mu_t */
+    if (P->factor != 1.0f) return 3;
     orc_layout L;
     orc_layout_compute(P, &L);
     const int32_t nt = P->ntissue;

@@ -147,6 +147,11 @@ int32_t orc_find_triangle(const float *tri, int32_t ntri, const float p[3],
 
 float orc_rspecular(const float *nmuasg, int32_t ntissue);
 float orc_fresnel(float n1, float n2, float cos_in);
+/* TransmitOrReflect (.cl:921-1018): replaces v (unit) by its direction after
+ * a boundary event at the unit normal nv from index n1 into n2 for the
+ * Fresnel draw x; returns ORC_EV_TRANSMIT or ORC_EV_REFLECT.  The photon
+ * loop calls this same function. */
+int orc_transmit_or_reflect(const float nv[3], float v[3], float n1, float n2, float x);
 
 /* One Henyey-Greenstein deflection of v (VScatter, .cl:597-655). */
 void orc_vscatter(float v[3], float g, float xi_theta, float xi_psi);
@@ -171,7 +176,8 @@ int orc_run(const orc_params *p, int64_t first, int64_t count,
  * the emitter frame (3-D DDA, Amanatides & Woo 1987).  Sums are 2^32 fixed
  * point, accumulated per (photon, detector) pair; photons [first,
  * first+count) are also run through the ordinary tallies of orc_run when
- * T is not NULL. */
+ * T is not NULL.  Returns 3 (refused) for a PixelPerUnit factor other than
+ * 1: the estimator assumes step lengths -ln(xi) / mu_t. */
 typedef struct orc_adjoint_out {
     float jac_origin[3];
     float jac_size[3];

@@ -63,7 +63,36 @@ def lib():
         if not os.path.exists(LIB_PATH):
             subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
                            capture_output=True)
-        L = C.CDLL(LIB_PATH)
+        _lib = _bind(C.CDLL(LIB_PATH))
+    return _lib
+
+
+class planted:
+    """Context manager: the oracle built with one planted mistake (the
+    ORC_PLANT_* macro `macro`, see mcml_oracle.c) into `tmpdir` stands in for
+    the real one, so a test can show that its checks catch that mistake."""
+
+    def __init__(self, macro, tmpdir):
+        self.path = os.path.join(str(tmpdir), "liboracle_%s.so" % macro.lower())
+        src = os.path.join(ROOT, "oracle", "mcml_oracle.c")
+        subprocess.run(["gcc", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-std=c11",
+                        "-shared", "-D" + macro, "-o", self.path, src, "-lm"], check=True, capture_output=True)
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = _bind(C.CDLL(self.path))
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
+
+
+def _bind(L):
+    """Declares the argument and result types of the oracle's functions."""
+    if True:
         L.orc_logf.restype = C.c_float
         L.orc_logf.argtypes = [C.c_float]
         L.orc_cosf.restype = C.c_float
@@ -85,6 +114,8 @@ def lib():
         L.orc_fresnel.restype = C.c_float
         L.orc_fresnel.argtypes = [C.c_float, C.c_float, C.c_float]
         L.orc_vscatter.argtypes = [_fp, C.c_float, C.c_float, C.c_float]
+        L.orc_transmit_or_reflect.restype = C.c_int
+        L.orc_transmit_or_reflect.argtypes = [_fp, _fp, C.c_float, C.c_float, C.c_float]
         L.orc_layout_compute.argtypes = [C.POINTER(Params), C.POINTER(Layout)]
         L.orc_run.restype = C.c_int
         L.orc_run.argtypes = [C.POINTER(Params), C.c_int64, C.c_int64, C.POINTER(Tallies), _fp, _fp,
@@ -93,8 +124,7 @@ def lib():
         L.orc_adjoint.argtypes = [C.POINTER(Params), C.c_int64, C.c_int64, C.POINTER(Tallies),
                                   C.POINTER(AdjointOut), C.c_int]
         L.orc_dda.argtypes = [C.POINTER(AdjointOut), _fp, C.c_float, _lp]
-        _lib = L
-    return _lib
+    return L
 
 
 def dda(seg, w, origin, size, n):
@@ -175,6 +205,14 @@ def find_triangle(tri, p, v, ref_tri=-1):
 
 def fresnel(n1, n2, c):
     return lib().orc_fresnel(n1, n2, c)
+
+
+def transmit_or_reflect(nv, v, n1, n2, x):
+    """orc_transmit_or_reflect: (kind, new direction)."""
+    a = np.ascontiguousarray(nv, np.float32).copy()
+    b = np.ascontiguousarray(v, np.float32).copy()
+    k = lib().orc_transmit_or_reflect(fptr(a), fptr(b), n1, n2, x)
+    return k, b
 
 
 def vscatter(v, g, xt, xp):

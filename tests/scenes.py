@@ -207,3 +207,46 @@ C0_LAYERS = [(0.15, 1.40, 0.4, 200.0, 0.9), (1.0, 1.44, 0.1, 120.0, 0.9), (2.0, 
 # unequal indices almost always reflects, which would keep the photons in
 # the top layers)
 SLAB10_LAYERS = [(0.1, 1.40, 0.05 + 0.08 * i, 15.0 + 3.0 * i, 0.80 + 0.015 * i) for i in range(10)]
+
+
+def escape_scene(n_up):
+    """models/model3 (open mesh of eight 10 x 10 cm faces) with two start
+    directions (GetvStart's vStartOcc counts, MCLightPropagation.cl:423-467):
+    the first n_up photons head +z into the layers as the fixture's emitter
+    does; the rest head away from the mesh (0.6, 0, -0.8), find no face ahead
+    and leave it (the reference's mediumIDs[-1] read, .cl:1475; counted as
+    escaped here)."""
+    sc = load_fixture("model3")
+    sc["vstart"] = np.array([[0.0, 0.0, 1.0], [0.6, 0.0, -0.8]], f32)
+    sc["vocc"] = np.array([n_up, 1 << 30], np.int32)
+    return sc
+
+
+# oblique photon through a weakly scattering slab at mismatched n
+OBLIQUE_D = 1.0
+OBLIQUE_N = (1.0, 1.5, 1.2)
+OBLIQUE_THETA = math.radians(40.0)
+OBLIQUE_X0, OBLIQUE_Z0 = 2.0, -0.5
+
+
+def oblique_slab():
+    """A 20 x 20 cm slab z in [0, OBLIQUE_D] (two quads, normals +z, so both
+    crossings of a photon heading down have cos_in > 0), tissue 1 weakly
+    scattering (n 1.5), behind it tissue 2 (n 1.2, mua = mus = 0: the photon
+    leaves there).  The source sits at (OBLIQUE_X0, 0, OBLIQUE_Z0), heading at
+    OBLIQUE_THETA from +z in the x-z plane."""
+    h, d = 10.0, OBLIQUE_D
+    tri, med = [], []
+    for z, pair in ((0.0, (0, 1)), (d, (1, 2))):
+        a, b, c, e = [-h, -h, z], [h, -h, z], [h, h, z], [-h, h, z]
+        tri += [a + b + c, a + c + e]
+        med += [pair, pair]
+    n0, n1, n2 = OBLIQUE_N
+    nm = [[n0, 0, 0, 0], [n1, 1e-6, 1e-6, 0.0], [n2, 0, 0, 0]]
+    v = [math.sin(OBLIQUE_THETA), 0.0, math.cos(OBLIQUE_THETA)]
+    return dict(tri=np.array(tri, f32).reshape(-1, 9), nrm=np.tile(np.array([0, 0, 1], f32), (4, 1)),
+                med=np.array(med, np.int32), nmuasg=np.array(nm, f32), fetal_id=-1,
+                pstart=np.array([OBLIQUE_X0, 0.0, OBLIQUE_Z0], f32), vstart=np.array([v], f32),
+                vocc=np.array([1], np.int32), poi_basis=np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1], f32),
+                model_dims=np.array([20.0, 20.0, 2 * d], f32), det_grid=np.array([0.1, 0.1, 0.1, 0.1, 1.0, 1.0], f32),
+                factor=f32(1.0), chance=f32(0.1), w_th=f32(1e-4))

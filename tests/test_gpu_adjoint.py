@@ -134,3 +134,36 @@ def test_scaling_identity_semi_infinite():
     print("sum mu dR/dmu: mean %.3e se %.3e scale %.4f R %.5f" % (vals.mean(), se, scale, np.mean(R)), flush=True)
     assert se < 0.01 * scale and se < 0.02 * np.mean(R)
     assert abs(vals.mean()) <= 4 * se + 1e-3 * scale
+
+
+def test_adjoint_outputs_are_cleared_when_nothing_is_detected():
+    # a call that fills the gradient and Jacobian buffers, then one whose
+    # detectors no photon reaches, then one with no photons: the later calls
+    # must return zeros (their buffers are cleared on the context's stream
+    # and copied back only after it has drained)
+    sc, lo, hi, grid = _scene("c0slab")
+    ctx = lp.Context(0)
+    ctx.set_scene(sc, seed=21)
+    ctx.set_detectors(lo, hi)
+    ctx.set_jacobian_grid(*grid)
+    first = ctx.adjoint(0, 4000, jac=True)
+    assert first["pairs"] > 0 and np.any(first["grad"] != 0) and np.any(first["jac"] != 0)
+    ctx.set_detectors([500.0, 600.0, 700.0], [501.0, 601.0, 701.0])
+    for count in (4000, 0):
+        r = ctx.adjoint(0, count, jac=True)
+        st = ctx.stats()
+        assert r["pairs"] == 0 and st.replay_mismatch == 0
+        assert not np.any(r["signals"]) and not np.any(r["grad"]) and not np.any(r["jac"])
+    ctx.close()
+
+
+def test_adjoint_refuses_a_pixel_per_unit_factor():
+    # the path-length estimator assumes step lengths -ln(xi) / mu_t
+    sc, lo, hi, grid = _scene("c0slab")
+    sc = dict(sc, factor=np.float32(2.0))
+    ctx = lp.Context(0)
+    ctx.set_scene(sc, seed=21)
+    ctx.set_detectors(lo, hi)
+    with pytest.raises(lp.LPError, match="factor"):
+        ctx.adjoint(0, 100)
+    ctx.close()

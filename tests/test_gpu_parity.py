@@ -43,7 +43,16 @@ def _scene(name):
         return S.maternal_fetal(freq_maternal=6, freq_fetal=6)
     if name == "model6FB":
         return scenes.load_fixture("model6", "configFB.xml")
+    if name == "planar":  # model4/configPlanar.xml: 20 planar faces, 160 x 160 cm
+        return scenes.load_fixture("model4", "configPlanar.xml")
+    if name == "escape":  # model3 with a second start direction that misses the mesh
+        return scenes.escape_scene(ESCAPE_UP)
+    if name == "oblique":  # oblique photon through a slab at mismatched n
+        return scenes.oblique_slab()
     return scenes.load_fixture(name)
+
+
+ESCAPE_UP = 150  # photons [0, 150) head into the mesh, the rest away from it
 
 
 def _configure(ctx, sc, seed, rng):
@@ -72,7 +81,8 @@ def test_math_core_bit_exact(ctx):
                           np.array([O.acosf(v) for v in cs], np.float32).view(np.uint32))
 
 
-CASES = [("model1", 512), ("model2", 512), ("model6FB", 256), ("c0slab", 256), ("model4", 48)]
+CASES = [("model1", 512), ("model2", 512), ("model6FB", 256), ("c0slab", 256), ("model4", 48), ("model3", 256),
+         ("planar", 128), ("escape", 256), ("oblique", 256)]
 
 
 @pytest.mark.parametrize("rng", [lp.RNG_PHILOX, lp.RNG_RAN3], ids=["philox", "ran3"])
@@ -94,10 +104,12 @@ def test_traces_bit_exact(ctx, name, n, rng):
     assert not bad, bad[:3]
 
 
+TALLY_CASES = [("model1", 20000), ("model2", 8000), ("c0slab", 3000), ("model4", 96), ("mf_coarse", 1500),
+               ("model3", 4000), ("planar", 1500), ("escape", 3000), ("oblique", 4000)]
+
+
 @pytest.mark.parametrize("rng", [lp.RNG_PHILOX, lp.RNG_RAN3], ids=["philox", "ran3"])
-@pytest.mark.parametrize("name,n", [("model1", 20000), ("model2", 8000), ("c0slab", 3000), ("model4", 96),
-                                    ("mf_coarse", 1500)],
-                         ids=["model1", "model2", "c0slab", "model4", "mf_coarse"])
+@pytest.mark.parametrize("name,n", TALLY_CASES, ids=[c[0] for c in TALLY_CASES])
 def test_tallies_bit_exact(ctx, name, n, rng):
     sc = _scene(name)
     _configure(ctx, sc, 99, rng)
@@ -113,6 +125,19 @@ def test_tallies_bit_exact(ctx, name, n, rng):
         assert np.array_equal(a, b), k
     for k in ("photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out", "nan"):
         assert gt["misc"][k] == ot["misc"][k], (k, gt["misc"][k], ot["misc"][k])
+    if name == "escape":  # the photons heading away found no face ahead and left
+        assert gt["misc"]["escaped"] == n - ESCAPE_UP
+    # with every search shortcut off (each step searched), the same results
+    if name in ("model4", "mf_coarse", "escape"):
+        ctx.set_option("no_skip", 1)
+        ctx.reset()
+        ctx.run(0, n)
+        assert np.array_equal(ctx.exit_records(n).view(np.uint32), oex.view(np.uint32))
+        g2 = ctx.tallies()
+        ctx.set_option("no_skip", 0)
+        assert g2["misc"]["skipped"] == 0
+        for k in ("sft", "rtd", "arz", "phirz", "region", "vox", "rad", "det"):
+            assert np.array_equal(g2[k], gt[k]), k
 
 
 def test_gpu_runs_are_reproducible_and_resumable(ctx):
@@ -193,7 +218,8 @@ def test_tuning_options_do_not_change_results():
     settings = [{}, {"near_r0": 0.0}, {"near_r0": 0.03}, {"near_r0": 0.08, "near_cell": 0.1},
                 {"free_iters": 4, "search_min": 0.5}, {"waves_per_simd": 1, "lds_nodes": 0},
                 {"dist_cap": 0.0}, {"clear_step": -1.0, "refill_min": 4}, {"clear_step": 0.05},
-                {"list_long": 0}, {"list_long": 0, "near_r0": 0.03, "dist_cap": 0.0}]
+                {"list_long": 0}, {"list_long": 0, "near_r0": 0.03, "dist_cap": 0.0}, {"no_skip": 1},
+                {"no_skip": 1, "near_r0": 0.0}]
     ref = None
     for opts in settings:
         ctx = lp.Context(0)

@@ -123,3 +123,13 @@ def test_gradients_equal_estimator_on_traces(layers):
     assert np.any(grad[:, 1:, 1] != 0)
     if layers == "slab10":
         assert np.all((grad[:, 1:, :] != 0).any(axis=0))  # every region reached by a detected path
+
+
+def test_oracle_adjoint_refuses_a_pixel_per_unit_factor():
+    import ctypes as C
+    sc = dict(scenes.slab(scenes.C0_LAYERS), factor=2.0)
+    run = O.Run(sc, seed=1, tallies={"det_rlo": [0.0], "det_rhi": [1.0]})
+    A = O.AdjointOut()
+    sig = np.zeros(1, np.int64)
+    A.signals = O.lptr(sig)
+    assert O.lib().orc_adjoint(C.byref(run.P), 0, 10, None, C.byref(A), 1) == 3

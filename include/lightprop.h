@@ -303,6 +303,22 @@ int lp_run_batch(lp_context *ctx, const float *nmuasg, int32_t ninst, int64_t fi
 int lp_run_batch_geom(lp_context *ctx, const float *nmuasg, int32_t ninst, const int32_t *geom, int64_t first,
                       int64_t count, int64_t *det, int64_t *region);
 
+/* lp_run_batch_geom that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The scene
+ * must already be on the device (an lp_run or lp_run_batch after the last
+ * change of mesh, source or search options; else LP_ESTATE).  Outputs go to
+ * the caller's device buffers, cleared by the call: det_dev[ninst][ndet][2]
+ * (may be NULL without detectors) and region_dev[ninst][ntissue], 2^32
+ * fixed point, bit-equal to lp_run_batch_geom's det and region.  Event
+ * counters are not kept.  The instance table and the work counter live in a
+ * static device array of 256 + 16 * 1152 bytes per GPU (held until the
+ * process ends), so ninst * (ntissue + 1) <= 1152 (else LP_EINVAL).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * threads, other contexts) race.  Blocks until done. */
+int lp_run_batch_noalloc(lp_context *ctx, const float *nmuasg, int32_t ninst, const int32_t *geom, int64_t first,
+                         int64_t count, int64_t *det_dev, int64_t *region_dev);
+
 /* Launch-option autotuner (no reference counterpart: the reference fixes
  * its launch at 6 400 work-items, CalculatePropagation.cpp:76): coordinate
  * search over waves per SIMD (the grid follows the variant's occupancy

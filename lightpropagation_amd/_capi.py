@@ -79,6 +79,8 @@ _SIGS = {
     "lp_run_batch": (C.c_int, [C.c_void_p, _fp, C.c_int32, C.c_int64, C.c_int64, _lp, _lp]),
     "lp_set_geometries": (C.c_int, [C.c_void_p, _lp, C.c_int32]),
     "lp_run_batch_geom": (C.c_int, [C.c_void_p, _fp, C.c_int32, _ip, C.c_int64, C.c_int64, _lp, _lp]),
+    "lp_run_batch_noalloc": (C.c_int, [C.c_void_p, _fp, C.c_int32, _ip, C.c_int64, C.c_int64, C.c_void_p,
+                                       C.c_void_p]),
     "lp_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(TuneResult)]),
     "lp_debug_math": (C.c_int, [C.c_void_p, C.c_int32, _fp, _fp, C.c_int64]),
     "lp_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),

@@ -97,6 +97,7 @@ struct lp_context {
     // with the kernel arguments and grid it was captured for
     DevBuf d_inst_mat, d_bdet, d_bregion;
     int32_t use_graph = 1;
+    void* scratch = nullptr;  // this device's static scratch (lp_run_batch_noalloc)
     hipGraph_t bgraph = nullptr;
     hipGraphExec_t bexec = nullptr;
     KArgs bargs{};
@@ -461,6 +462,8 @@ int lp_create(lp_context** out, int32_t device)
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    // the device address of this GPU's static scratch (device current)
+    if (static_scratch(device, &c->scratch)) c->scratch = nullptr;
     *out = c;
     return LP_OK;
 }
@@ -755,6 +758,40 @@ int lp_run(lp_context* c, int64_t first, int64_t count)
 
 namespace {
 
+// Checks a batched launch's arguments and writes its material table: per
+// instance the ntissue rows (n, mua, mus, g), then (rsp, BVH root bits, cell
+// offset bits, 0) of its geometry; ninst * (ntissue + 1) float4 rows.
+int build_inst_table(lp_context* c, const float* nmuasg, int32_t ninst, const int32_t* geom, int64_t count,
+                     const char* who, float* tab)
+{
+    const int32_t ngeom = (int32_t)c->part_root.size();
+    if (geom && ngeom > 1 && c->rng_mode != 0)
+        return fail(c, LP_EINVAL, "%s: several geometries need the Philox streams (rng_mode 0)", who);
+    if (geom)
+        for (int32_t i = 0; i < ninst; ++i)
+            if (geom[i] < 0 || geom[i] >= ngeom)
+                return fail(c, LP_EINVAL, "%s: instance %d: geometry %d outside [0, %d)", who, i, geom[i], ngeom);
+    const int32_t nt = c->ntissue;
+    if ((int64_t)ninst * (nt + 1) > kMaxInstRows)
+        return fail(c, LP_EINVAL, "%s: %d instances of %d tissues exceed the %d-row material table", who, ninst, nt,
+                    kMaxInstRows);
+    if (count > 0 && (int64_t)ninst > INT64_MAX / 2 / count) return fail(c, LP_EINVAL, "%s: too many photons", who);
+    for (int32_t i = 0; i < ninst; ++i) {
+        const float* m = nmuasg + (size_t)i * nt * 4;
+        for (int32_t t = 0; t < nt; ++t)
+            if (!(m[4 * t] > 0) || !(m[4 * t + 1] >= 0) || !(m[4 * t + 2] >= 0) || !(m[4 * t + 3] >= -1 && m[4 * t + 3] <= 1))
+                return fail(c, LP_EINVAL, "%s: instance %d tissue %d: need n > 0, mua, mus >= 0, |g| <= 1", who, i, t);
+        std::memcpy(&tab[(size_t)i * (nt + 1) * 4], m, (size_t)nt * 4 * sizeof(float));
+        float* row = &tab[((size_t)i * (nt + 1) + nt) * 4];
+        row[0] = rspecular(m, nt);
+        row[3] = 0.0f;
+        const int32_t g = geom ? geom[i] : 0;
+        std::memcpy(&row[1], &c->part_root[g], sizeof(int32_t));
+        std::memcpy(&row[2], &c->part_coff[g], sizeof(uint32_t));
+    }
+    return LP_OK;
+}
+
 // Captures [zero the instance tallies, zero the work counter, transport
 // launch] on the context's stream as a graph, unless the graph held was
 // captured for the same kernel arguments and grid.  Memory is cleared by a
@@ -809,32 +846,11 @@ int lp_run_batch_geom(lp_context* c, const float* nmuasg, int32_t ninst, const i
     if (!nmuasg || ninst < 1 || first < 0 || count < 0) return fail(c, LP_EINVAL, "lp_run_batch: bad arguments");
     int rc = prepare(c);
     if (rc) return rc;
-    const int32_t ngeom = (int32_t)c->part_root.size();
-    if (geom && ngeom > 1 && c->rng_mode != 0)
-        return fail(c, LP_EINVAL, "lp_run_batch: several geometries need the Philox streams (rng_mode 0)");
-    if (geom)
-        for (int32_t i = 0; i < ninst; ++i)
-            if (geom[i] < 0 || geom[i] >= ngeom)
-                return fail(c, LP_EINVAL, "lp_run_batch: instance %d: geometry %d outside [0, %d)", i, geom[i], ngeom);
     const int32_t nt = c->ntissue, nd = (int32_t)c->det_rlo.size();
-    if ((int64_t)ninst * (nt + 1) > kMaxInstRows)
-        return fail(c, LP_EINVAL, "lp_run_batch: %d instances of %d tissues exceed the %d-row material table", ninst,
-                    nt, kMaxInstRows);
-    if (count > 0 && (int64_t)ninst > INT64_MAX / 2 / count) return fail(c, LP_EINVAL, "lp_run_batch: too many photons");
-    const int64_t ndw = (int64_t)ninst * nd * 2, nrw = (int64_t)ninst * nt;
     std::vector<float> tab((size_t)ninst * (nt + 1) * 4, 0.0f);
-    for (int32_t i = 0; i < ninst; ++i) {
-        const float* m = nmuasg + (size_t)i * nt * 4;
-        for (int32_t t = 0; t < nt; ++t)
-            if (!(m[4 * t] > 0) || !(m[4 * t + 1] >= 0) || !(m[4 * t + 2] >= 0) || !(m[4 * t + 3] >= -1 && m[4 * t + 3] <= 1))
-                return fail(c, LP_EINVAL, "lp_run_batch: instance %d tissue %d: need n > 0, mua, mus >= 0, |g| <= 1", i, t);
-        std::memcpy(&tab[(size_t)i * (nt + 1) * 4], m, (size_t)nt * 4 * sizeof(float));
-        float* row = &tab[((size_t)i * (nt + 1) + nt) * 4];  // (rsp, root bits, cell offset bits, 0)
-        row[0] = rspecular(m, nt);
-        const int32_t g = geom ? geom[i] : 0;
-        std::memcpy(&row[1], &c->part_root[g], sizeof(int32_t));
-        std::memcpy(&row[2], &c->part_coff[g], sizeof(uint32_t));
-    }
+    if ((rc = build_inst_table(c, nmuasg, ninst, geom, count, "lp_run_batch", tab.data()))) return rc;
+    const int32_t ngeom = (int32_t)c->part_root.size();
+    const int64_t ndw = (int64_t)ninst * nd * 2, nrw = (int64_t)ninst * nt;
     if ((rc = dev_alloc(c, c->d_inst_mat, tab.size() * sizeof(float)))) return rc;
     if ((rc = dev_alloc(c, c->d_bdet, (size_t)std::max<int64_t>(ndw, 1) * sizeof(int64_t)))) return rc;
     if ((rc = dev_alloc(c, c->d_bregion, (size_t)nrw * sizeof(int64_t)))) return rc;
@@ -877,6 +893,72 @@ int lp_run_batch_geom(lp_context* c, const float* nmuasg, int32_t ninst, const i
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (det && ndw) HIPCHK(c, hipMemcpy(det, c->d_bdet.p, (size_t)ndw * sizeof(int64_t), hipMemcpyDeviceToHost));
     if (region) HIPCHK(c, hipMemcpy(region, c->d_bregion.p, (size_t)nrw * sizeof(int64_t), hipMemcpyDeviceToHost));
+    c->stats.kernel_ms = ms;
+    c->stats.photons = a.count;
+    c->stats.grid_blocks = grid;
+    return LP_OK;
+}
+
+int lp_run_batch_noalloc(lp_context* c, const float* nmuasg, int32_t ninst, const int32_t* geom, int64_t first,
+                         int64_t count, int64_t* det_dev, int64_t* region_dev)
+{
+    if (!c) return LP_EINVAL;
+    if (!nmuasg || ninst < 1 || first < 0 || count < 0 || !region_dev)
+        return fail(c, LP_EINVAL, "lp_run_batch_noalloc: bad arguments");
+    if (!c->scratch) return fail(c, LP_EDEVICE, "lp_run_batch_noalloc: no static scratch on device %d", c->device);
+    // nothing may be built or uploaded here: the scene must already be on the device
+    if (c->ntri < 0 || c->mesh_dirty || c->src_dirty || !c->d_counter.p)
+        return fail(c, LP_ESTATE,
+                    "lp_run_batch_noalloc: the scene is not on the device (run lp_run or lp_run_batch once after "
+                    "the last change of mesh, source or search options)");
+    int rc = check_ready(c);
+    if (rc) return rc;
+    const int32_t nt = c->ntissue, nd = (int32_t)c->det_rlo.size();
+    if (nd > 0 && !det_dev) return fail(c, LP_EINVAL, "lp_run_batch_noalloc: detectors are set but det_dev is NULL");
+    if ((int64_t)ninst * (nt + 1) > kMaxInstRows)
+        return fail(c, LP_EINVAL,
+                    "lp_run_batch_noalloc: %d instances of %d tissues need %lld material rows; the static table "
+                    "holds %d (%zu bytes)",
+                    ninst, nt, (long long)ninst * (nt + 1), kMaxInstRows, (size_t)16 * kMaxInstRows);
+    float tab[4 * kMaxInstRows];  // host staging of the table (stack, no allocation)
+    if ((rc = build_inst_table(c, nmuasg, ninst, geom, count, "lp_run_batch_noalloc", tab))) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    compute_layout(c);
+    const int32_t ngeom = (int32_t)c->part_root.size();
+    const int64_t ndw = (int64_t)ninst * nd * 2, nrw = (int64_t)ninst * nt;
+    char* sp = (char*)c->scratch;
+    KArgs a;
+    fill_args(c, a);
+    a.sft = a.rtd = a.arz = a.phirz = a.vox = a.rad = nullptr;
+    a.misc = nullptr;  // no event counters: the context's tallies are not touched
+    a.det = nd > 0 ? (unsigned long long*)det_dev : nullptr;
+    a.region = (unsigned long long*)region_dev;
+    a.counter = (unsigned long long*)sp;
+    a.inst_mat = (const float*)(sp + kScratchCounterBytes);
+    a.ninst = ninst;
+    a.ngeom = geom ? ngeom : 1;
+    a.lmat_n = std::max<int32_t>(kMaxTissues + 1, ninst * (nt + 1));
+    a.inst_count = std::max<int64_t>(count, 1);
+    a.first = first;
+    a.count = (int64_t)ninst * count;
+    const int grid = grid_for(c, a);
+    // the scratch keeps the last call's contents: clear the counter (a fill
+    // kernel, not a memset node) and the caller's outputs, then the table
+    int bad = launch_zero_u64(a.counter, 8, c->stream);
+    if (a.det) bad |= launch_zero_u64(a.det, ndw, c->stream);
+    bad |= launch_zero_u64(a.region, nrw, c->stream);
+    if (bad) return fail(c, LP_EDEVICE, "lp_run_batch_noalloc: clearing the outputs failed");
+    HIPCHK(c, hipMemcpyAsync(sp + kScratchCounterBytes, tab, (size_t)ninst * (nt + 1) * 4 * sizeof(float),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (a.count > 0) {
+        char err[256] = {0};
+        if (launch_transport(a, grid, c->stream, err, sizeof err)) return fail(c, LP_EDEVICE, "%s", err);
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->stats.kernel_ms = ms;
     c->stats.photons = a.count;
     c->stats.grid_blocks = grid;

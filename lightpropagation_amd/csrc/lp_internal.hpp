@@ -241,4 +241,14 @@ constexpr int kMaxInstRows = 1152;
 // geometries one mesh may hold (lp_set_geometries)
 constexpr int kMaxGeoms = 64;
 
+// Static scratch of the allocation-free batched launch, one per GPU for the
+// life of the process: [0, kScratchCounterBytes) the work counter, then the
+// instance material table (kMaxInstRows float4 rows).
+constexpr size_t kScratchCounterBytes = 256;
+constexpr size_t kStaticScratchBytes = kScratchCounterBytes + 16 * (size_t)kMaxInstRows;
+constexpr int kMaxDevices = 64;
+// Device address of that scratch on `device`, which must be current; looked
+// up once per device and kept.  Returns nonzero on failure.
+int static_scratch(int device, void** out);
+
 }  // namespace lp

@@ -20,6 +20,8 @@
 //  * all tallies are 64-bit fixed-point integer atomics.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "lp_devmath.hpp"
 #include "lp_internal.hpp"
 
@@ -1400,6 +1402,32 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             atomic_add_u64(&a.misc[kMiscReplayMismatch], wc[wMismatch]);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// static scratch of the allocation-free batched launch
+// ---------------------------------------------------------------------------
+// Each GPU holds its own copy until the process ends.  Host code uses the
+// address hipGetSymbolAddress returns for the current device, never the
+// array's name (a host-side stand-in that is not a device address).
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int static_scratch(int device, void** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices || !out) return 1;
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return 1;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
 }
 
 namespace {

@@ -203,7 +203,7 @@ def adjoint_vs_fd(sc, photons=10_000_000, batches=4, rels=(1e-3, 2e-2), jac_n=64
 # fits under them) x 2 wavelengths x 16 fetal SpO2 in [0.30, 0.75] (maternal
 # 0.97) = 128 instances: all in one batched launch over the merged
 # geometries, and for comparison one launch of 32 instances per geometry.
-C2_GEOMS = [(1.5, 19, {"t_fat": 0.3, "t_uterus": 0.5}), (2.5, 38, {}), (3.5, 45, {}), (4.5, 53, {})]
+C2_GEOMS = [(1.5, 19, 14, {"t_fat": 0.3, "t_uterus": 0.5}), (2.5, 38, 27, {}), (3.5, 45, 32, {}), (4.5, 53, 38, {})]
 
 
 def batch_c2(photons=1_000_000, tune=True):
@@ -215,8 +215,8 @@ def batch_c2(photons=1_000_000, tune=True):
     spo2 = np.linspace(0.30, 0.75, 16)
     lo, hi = detectors()
     out, total_ph, total_s = [], 0, 0.0
-    for depth, freq, kw in C2_GEOMS:
-        sc = scenes.maternal_fetal(fetal_depth=depth, freq_maternal=freq, freq_fetal=freq, **kw)
+    for depth, fm, ff, kw in C2_GEOMS:
+        sc = scenes.maternal_fetal(fetal_depth=depth, freq_maternal=fm, freq_fetal=ff, **kw)
         tab = np.stack([scenes.mf_optics(wl, 0.97, sf) for wl in (660.0, 940.0) for sf in spo2])
         ctx = lp.Context(0)
         ctx.set_scene(sc, seed=0x5EED0002)
@@ -259,7 +259,7 @@ def batch_c2_one_launch(photons=1_000_000, tune=True):
     from lightpropagation_amd import scenes
     spo2 = np.linspace(0.30, 0.75, 16)
     lo, hi = detectors()
-    scs = [scenes.maternal_fetal(fetal_depth=d, freq_maternal=f, freq_fetal=f, **kw) for d, f, kw in C2_GEOMS]
+    scs = [scenes.maternal_fetal(fetal_depth=d, freq_maternal=fm, freq_fetal=ff, **kw) for d, fm, ff, kw in C2_GEOMS]
     merged, counts = scenes.geometries(scs)
     one = np.stack([scenes.mf_optics(wl, 0.97, sf) for wl in (660.0, 940.0) for sf in spo2])
     tab = np.concatenate([one] * len(scs))
@@ -296,12 +296,12 @@ def batch_c2_one_launch(photons=1_000_000, tune=True):
 
 def large_c4(photons=100_000_000):
     """The large workload: frequency-118 shells plus a frequency-40 fetal
-    torso ellipsoid (1 981 360 faces), 940 nm, a 512^3 int64 absorption grid
+    torso ellipsoid (1 992 320 faces, 10 regions), 940 nm, a 512^3 int64 absorption grid
     (1 GiB) and 8 detectors; photons/s of the forward pass, then the adjoint
     (forward + replay, 7 tissues x (mua, mus)) of the same photons."""
     import lightpropagation_amd as lp
     from lightpropagation_amd import scenes
-    sc = scenes.maternal_fetal(wavelength_nm=940.0, freq_maternal=118, freq_fetal=118, freq_torso=40)
+    sc = scenes.maternal_fetal(wavelength_nm=940.0, freq_maternal=118, freq_fetal=92, freq_torso=40)
     ctx = lp.Context(0)
     t0 = time.perf_counter()
     ctx.set_scene(sc, seed=0x5EED0004)
@@ -325,7 +325,7 @@ def large_c4(photons=100_000_000):
     adj_s = time.perf_counter() - t0
     sa = ctx.stats()
     ctx.close()
-    return {"workload": "1 981 360 faces (shells + fetal torso), 940 nm, 512^3 int64 grid + 8 detectors, %d photons"
+    return {"workload": "1 992 320 faces (shells + fetal torso), 940 nm, 512^3 int64 grid + 8 detectors, %d photons"
                         % photons,
             "setup_s": setup_s, "photons_per_s": photons / fwd_s, "kernel_ms": st.kernel_ms,
             "steps_per_photon": m["steps"] / photons, "device_bytes": st.device_bytes,

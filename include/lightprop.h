@@ -198,6 +198,15 @@ int lp_get_stats(lp_context *ctx, lp_stats *out);
 int lp_set_jacobian_grid(lp_context *ctx, const float origin[3], const float size[3], const int32_t n[3]);
 int lp_adjoint(lp_context *ctx, int64_t first, int64_t count, int64_t *signals, int64_t *grad, int64_t *jac,
                int64_t *npairs);
+/* lp_adjoint with the per-voxel mus Jacobian as well (no reference
+ * counterpart): jac_mus[ndet][nvox] = dS_d / dmus_v, the mus of the tissue
+ * in voxel v perturbed there only, by the same estimator:
+ *   dW/dmus_v = (sum over the interactions in v of 1 / mus) W - L_v W,
+ * 2^32 fixed point, accumulated as the mua term (jac) plus W / mus of each
+ * interaction in the voxel, so jac_mus - jac is the interaction term alone.
+ * jac or jac_mus may be NULL. */
+int lp_adjoint_voxels(lp_context *ctx, int64_t first, int64_t count, int64_t *signals, int64_t *grad, int64_t *jac,
+                      int64_t *jac_mus, int64_t *npairs);
 
 /* Evaluates the kernel's math core element-wise on the device (op 0 sqrt,
  * 1 divide, 2 log, 3 cos, 4 acos, 5 multiply, 6 add) over n (x, y) pairs.
@@ -292,7 +301,7 @@ int lp_set_option(lp_context *ctx, int32_t key, double value);
  * context's own tallies are untouched; event counters add to LP_TALLY_MISC.
  * The launch (tally clears + transport) is captured once as a HIP graph and
  * replayed while the scene, options and (ninst, first, count) stay the same;
- * new optics only refill the device table.  ninst * (ntissue + 1) <= 1152. */
+ * new optics only refill the device table.  ninst * (ntissue + 1) <= 2048. */
 int lp_run_batch(lp_context *ctx, const float *nmuasg, int32_t ninst, int64_t first, int64_t count, int64_t *det,
                  int64_t *region);
 
@@ -312,8 +321,8 @@ int lp_run_batch_geom(lp_context *ctx, const float *nmuasg, int32_t ninst, const
  * (may be NULL without detectors) and region_dev[ninst][ntissue], 2^32
  * fixed point, bit-equal to lp_run_batch_geom's det and region.  Event
  * counters are not kept.  The instance table and the work counter live in a
- * static device array of 256 + 16 * 1152 bytes per GPU (held until the
- * process ends), so ninst * (ntissue + 1) <= 1152 (else LP_EINVAL).  Every
+ * static device array of 256 + 16 * 2048 bytes per GPU (held until the
+ * process ends), so ninst * (ntissue + 1) <= 2048 (else LP_EINVAL).  Every
  * call on one GPU shares that array: calls that overlap on one GPU (other
  * threads, other contexts) race.  Blocks until done. */
 int lp_run_batch_noalloc(lp_context *ctx, const float *nmuasg, int32_t ninst, const int32_t *geom, int64_t first,

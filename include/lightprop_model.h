@@ -21,11 +21,15 @@ extern "C" {
 /* Maternal-fetal abdomen (synthetic benchmark scene, no reference
  * counterpart): concentric maternal spheres (skin, fat, uterine
  * wall, then the amniotic cavity) tangent to z = 0 at the origin, centred at
- * (0, 0, r_maternal), over a fetal head of three concentric spheres (scalp,
- * skull, brain) whose top lies at depth `fetal_depth` on the z axis.
- * Tissue ids: 0 ambient, 1 maternal skin, 2 fat, 3 uterine wall, 4 amniotic
- * fluid, 5 fetal scalp, 6 fetal skull, 7 fetal brain.  Each sphere is a
- * geodesic icosphere of frequency `freq` (20 freq^2 faces), watertight. */
+ * (0, 0, r_maternal), over a fetal head of concentric spheres whose top lies
+ * at depth `fetal_depth` on the z axis, and optionally a fetal torso.
+ * regions = 10 (default): tissue ids 0 ambient, 1 maternal skin, 2 fat,
+ * 3 uterine wall, 4 amniotic fluid, 5 fetal scalp, 6 fetal skull, 7 CSF,
+ * 8 grey matter, 9 white matter, 10 fetal body (the torso).
+ * regions = 7: 0-6 as above, 7 fetal brain (no CSF / grey / white shells),
+ * the torso (if any) of the scalp's tissue 5.
+ * Each sphere is a geodesic icosphere of frequency `freq` (20 freq^2
+ * faces), watertight. */
 typedef struct lpm_mf_params {
     double r_maternal;   /* outer abdomen radius [cm] (15)                      */
     double t_skin, t_fat, t_uterus; /* maternal layer thicknesses (0.2, 1.0, 0.8) */
@@ -39,17 +43,21 @@ typedef struct lpm_mf_params {
     int32_t freq_torso;
     double torso_axes[3];           /* semi-axes along x, y, z (4.5, 4.0, 6.0)  */
     double torso_center[3];         /* centre (0, 0, 18.0): below the head      */
+    int32_t regions;                /* 10 (default) or 7, see above             */
+    double t_csf, t_grey;           /* CSF and grey matter thicknesses (0.2, 0.4) */
 } lpm_mf_params;
 
-/* Fills *p with the defaults above and a frequency of 38 for all seven
- * spheres (202 160 faces), no torso. */
+/* Fills *p with the defaults above: 10 regions, maternal spheres of
+ * frequency 38, the five fetal head spheres of frequency 27, a torso of
+ * frequency 20 (196 420 faces). */
 void lpm_mf_default(lpm_mf_params *p);
 int64_t lpm_mf_num_triangles(const lpm_mf_params *p);
 int lpm_mf_build(const lpm_mf_params *p, float *tri, float *nrm, int32_t *med, int64_t ntri);
 
-/* Optical properties (n, mua, mus, g) of the 8 maternal-fetal tissues at
+/* Optical properties (n, mua, mus, g) of the maternal-fetal tissues at
  * 660 or 940 nm (the two rows of the haemoglobin table; other wavelengths
- * return 3) for maternal and fetal arterial oxygen saturations. */
+ * return 3) for maternal and fetal arterial oxygen saturations; ntissue 11
+ * (regions 10) or 8 (regions 7). */
 int lpm_mf_optics(double wavelength_nm, double spo2_maternal, double spo2_fetal, float *nmuasg, int32_t ntissue);
 
 /* Planar layered slab in the reference convention: x, y in [-hx, hx] x

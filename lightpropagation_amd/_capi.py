@@ -76,6 +76,7 @@ _SIGS = {
     "lp_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "lp_set_jacobian_grid": (C.c_int, [C.c_void_p, _fp, _fp, _ip]),
     "lp_adjoint": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _lp, _lp, _lp, _lp]),
+    "lp_adjoint_voxels": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _lp, _lp, _lp, _lp, _lp]),
     "lp_run_batch": (C.c_int, [C.c_void_p, _fp, C.c_int32, C.c_int64, C.c_int64, _lp, _lp]),
     "lp_set_geometries": (C.c_int, [C.c_void_p, _lp, C.c_int32]),
     "lp_run_batch_geom": (C.c_int, [C.c_void_p, _fp, C.c_int32, _ip, C.c_int64, C.c_int64, _lp, _lp]),
@@ -254,10 +255,11 @@ class Context:
         self._chk(lib().lp_set_jacobian_grid(self._c, fp(_f(origin)), fp(_f(size)), ip(_i(n))),
                   "lp_set_jacobian_grid")
 
-    def adjoint(self, first, count, jac=False, ntissue=None, ndet=None):
-        """lp_adjoint: dict(signals (ndet,), grad (ndet, ntissue, 2),
-        jac (ndet, nz, ny, nx) or None, pairs) as int64 fixed point
-        (2^32 per unit); the photons also add to the tallies."""
+    def adjoint(self, first, count, jac=False, ntissue=None, ndet=None, jac_mus=False):
+        """lp_adjoint (lp_adjoint_voxels with jac_mus): dict(signals (ndet,),
+        grad (ndet, ntissue, 2), jac (ndet, nz, ny, nx) or None, jac_mus
+        (same shape) or None, pairs) as int64 fixed point (2^32 per unit);
+        the photons also add to the tallies."""
         nt = ntissue if ntissue is not None else self._keep["ntissue"]
         nd = ndet if ndet is not None else self._keep["ndet"]
         sig = np.zeros(nd, np.int64)
@@ -266,11 +268,16 @@ class Context:
         if jac:
             nx, ny, nz = getattr(self, "_jac_n", (0, 0, 0))
             J = np.zeros((nd, nz, ny, nx), np.int64)
+        Js = None
+        if jac_mus:
+            nx, ny, nz = getattr(self, "_jac_n", (0, 0, 0))
+            Js = np.zeros((nd, nz, ny, nx), np.int64)
         pairs = C.c_int64()
-        self._chk(lib().lp_adjoint(self._c, int(first), int(count), sig.ctypes.data_as(_lp),
-                                   grad.ctypes.data_as(_lp), J.ctypes.data_as(_lp) if J is not None else None,
-                                   C.byref(pairs)), "lp_adjoint")
-        return dict(signals=sig, grad=grad, jac=J, pairs=pairs.value)
+        self._chk(lib().lp_adjoint_voxels(self._c, int(first), int(count), sig.ctypes.data_as(_lp),
+                                          grad.ctypes.data_as(_lp), J.ctypes.data_as(_lp) if J is not None else None,
+                                          Js.ctypes.data_as(_lp) if Js is not None else None, C.byref(pairs)),
+                  "lp_adjoint_voxels")
+        return dict(signals=sig, grad=grad, jac=J, jac_mus=Js, pairs=pairs.value)
 
     def run_batch(self, nmuasg, first, count, geom=None):
         """lp_run_batch (geom None) or lp_run_batch_geom (geom: geometry of

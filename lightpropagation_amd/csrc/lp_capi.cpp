@@ -88,7 +88,7 @@ struct lp_context {
     DevBuf d_tris, d_nodes, d_nrm, d_med, d_safety, d_vstart, d_vocc, d_counter, d_exit;
     DevBuf d_tally[LP_NTALLY];
     DevBuf d_tune[LP_NTALLY];  // the autotuner's stand-ins for the tallies (same sizes)
-    DevBuf d_det_list, d_det_count, d_grad, d_jac, d_adj_misc;
+    DevBuf d_det_list, d_det_count, d_grad, d_jac, d_jac_coll, d_adj_misc;
     int64_t exit_count = 0;
     lp_stats stats{};
 
@@ -249,7 +249,7 @@ int prepare(lp_context* c)
         if (c->part_first.size() > 2) {
             if (cells > 0) return fail(c, LP_EINVAL, "the safety grid needs a single geometry");
             const int ng = (int)c->part_first.size() - 1;
-            const int lds = c->lds_override >= 0 ? c->lds_override : 512;
+            const int lds = c->lds_override >= 0 ? c->lds_override : kMultiLdsNodes;
             MultiMesh mm;
             std::string why;
             if (!build_multi_mesh(c->tri.data(), c->med.data(), c->part_first, std::max(1, lds / ng), c->near_r0, cell,
@@ -308,7 +308,9 @@ void fill_args(lp_context* c, KArgs& a)
     a.tris = (const TriRec*)c->d_tris.p;
     a.nodes = (const BvhNode*)c->d_nodes.p;
     a.nnodes = (int32_t)c->bvh.nodes.size();
-    int lds = c->lds_override >= 0 ? c->lds_override : 512;
+    // a launch over several geometries stages fewer BVH nodes, leaving LDS
+    // for the larger material tables of its many instances
+    int lds = c->lds_override >= 0 ? c->lds_override : (c->part_root.size() > 1 ? kMultiLdsNodes : 512);
     a.nlds = std::min<int32_t>(a.nnodes, lds);
     a.nrm = (const float*)c->d_nrm.p;
     a.med = (const int32_t*)c->d_med.p;
@@ -477,8 +479,8 @@ void lp_destroy(lp_context* c)
     if (c->bgraph) hipGraphDestroy(c->bgraph);
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
                       &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list,
-                      &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_adj_misc, &c->d_inst_mat, &c->d_bdet,
-                      &c->d_bregion};
+                      &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_jac_coll, &c->d_adj_misc, &c->d_inst_mat,
+                      &c->d_bdet, &c->d_bregion};
     for (DevBuf* b : bufs) dev_free(*b);
     for (auto& b : c->d_tally) dev_free(b);
     for (auto& b : c->d_tune) dev_free(b);
@@ -1192,6 +1194,12 @@ int lp_set_jacobian_grid(lp_context* c, const float origin[3], const float size[
 int lp_adjoint(lp_context* c, int64_t first, int64_t count, int64_t* signals, int64_t* grad, int64_t* jac,
                int64_t* npairs)
 {
+    return lp_adjoint_voxels(c, first, count, signals, grad, jac, nullptr, npairs);
+}
+
+int lp_adjoint_voxels(lp_context* c, int64_t first, int64_t count, int64_t* signals, int64_t* grad, int64_t* jac,
+                      int64_t* jac_mus, int64_t* npairs)
+{
     if (!c) return LP_EINVAL;
     if (first < 0 || count < 0) return fail(c, LP_EINVAL, "lp_adjoint: negative range");
     int rc = prepare(c);
@@ -1248,12 +1256,17 @@ int lp_adjoint(lp_context* c, int64_t first, int64_t count, int64_t* signals, in
 
     // 2. replay the detected photons, accumulating the gradients
     const size_t gbytes = (size_t)nd * nt * 2 * sizeof(int64_t);
-    const size_t jbytes = (jac && nvox > 0) ? (size_t)nd * nvox * sizeof(int64_t) : 0;
+    const size_t jbytes = ((jac || jac_mus) && nvox > 0) ? (size_t)nd * nvox * sizeof(int64_t) : 0;
+    const bool want_mus = jac_mus && jbytes;
     if ((rc = dev_alloc(c, c->d_grad, gbytes))) return rc;
     HIPCHK(c, hipMemsetAsync(c->d_grad.p, 0, gbytes, c->stream));
     if (jbytes) {
         if ((rc = dev_alloc(c, c->d_jac, jbytes))) return rc;
         HIPCHK(c, hipMemsetAsync(c->d_jac.p, 0, jbytes, c->stream));
+    }
+    if (want_mus) {
+        if ((rc = dev_alloc(c, c->d_jac_coll, jbytes))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->d_jac_coll.p, 0, jbytes, c->stream));
     }
     if ((rc = dev_alloc(c, c->d_adj_misc, kMiscN * sizeof(int64_t)))) return rc;
     HIPCHK(c, hipMemsetAsync(c->d_adj_misc.p, 0, kMiscN * sizeof(int64_t), c->stream));
@@ -1267,6 +1280,7 @@ int lp_adjoint(lp_context* c, int64_t first, int64_t count, int64_t* signals, in
     b.adj_list = (const DetRec*)c->d_det_list.p;
     b.grad = (unsigned long long*)c->d_grad.p;
     b.jac = jbytes ? (unsigned long long*)c->d_jac.p : nullptr;
+    b.jac_coll = want_mus ? (unsigned long long*)c->d_jac_coll.p : nullptr;
     for (int k = 0; k < 3; ++k) {
         b.jac_origin[k] = c->jac_origin[k];
         b.jac_inv[k] = 1.0f / c->jac_size[k];
@@ -1280,8 +1294,19 @@ int lp_adjoint(lp_context* c, int64_t first, int64_t count, int64_t* signals, in
     int64_t misc[kMiscN];
     HIPCHK(c, hipMemcpyAsync(misc, c->d_adj_misc.p, sizeof misc, hipMemcpyDeviceToHost, c->stream));
     if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->d_grad.p, gbytes, hipMemcpyDeviceToHost, c->stream));
-    if (jbytes) HIPCHK(c, hipMemcpyAsync(jac, c->d_jac.p, jbytes, hipMemcpyDeviceToHost, c->stream));
+    if (jbytes && jac) HIPCHK(c, hipMemcpyAsync(jac, c->d_jac.p, jbytes, hipMemcpyDeviceToHost, c->stream));
+    if (want_mus) {  // d/dmus of a voxel = the mua term (-L_v W) + its interactions' W / mus
+        HIPCHK(c, hipMemcpyAsync(jac_mus, c->d_jac_coll.p, jbytes, hipMemcpyDeviceToHost, c->stream));
+        if (!jac) {
+            std::vector<int64_t> ja((size_t)nd * nvox);
+            HIPCHK(c, hipMemcpyAsync(ja.data(), c->d_jac.p, jbytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (size_t i = 0; i < ja.size(); ++i) jac_mus[i] += ja[i];
+        }
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (want_mus && jac)
+        for (int64_t i = 0; i < (int64_t)nd * nvox; ++i) jac_mus[i] += jac[i];
     if (npairs) *npairs = (int64_t)n;
     c->stats.kernel_ms = fwd_ms;
     c->stats.photons = count;
@@ -1297,8 +1322,8 @@ int lp_get_stats(lp_context* c, lp_stats* out)
     int64_t bytes = 0;
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
                       &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list,
-                      &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_adj_misc, &c->d_inst_mat, &c->d_bdet,
-                      &c->d_bregion};
+                      &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_jac_coll, &c->d_adj_misc, &c->d_inst_mat,
+                      &c->d_bdet, &c->d_bregion};
     for (DevBuf* b : bufs) bytes += (int64_t)b->bytes;
     for (auto& b : c->d_tally) bytes += (int64_t)b.bytes;
     c->stats.device_bytes = bytes;

@@ -190,6 +190,8 @@ struct KArgs {
     const struct DetRec* adj_list;
     unsigned long long* grad;  // ndet * ntissue * 2: (d/dmua, d/dmus)
     unsigned long long* jac;   // ndet * jac voxels (null = no voxel Jacobian)
+    unsigned long long* jac_coll;  // ndet * jac voxels: W / mus of each interaction in the voxel
+                                   // (null = none); d/dmus of voxel v = jac[v] + jac_coll[v]
     float jac_origin[3], jac_inv[3];  // jac_inv = 1 / voxel size in float
     int32_t jac_n[3];
     // batched launch (inst_mat null = one instance with nmuasg and rsp):
@@ -237,7 +239,10 @@ int launch_debug_math(int op, const float* in, float* out, int64_t n, void* stre
 int launch_debug_copy(const void* src, void* dst, int64_t bytes, void* stream);
 int launch_zero_u64(void* p, int64_t n, void* stream);
 // rows of the per-instance material table a batched launch may stage in LDS
-constexpr int kMaxInstRows = 1152;
+constexpr int kMaxInstRows = 2048;
+// BVH nodes staged in LDS by default in a mesh of several geometries (split
+// evenly between their trees); 512 with one geometry
+constexpr int kMultiLdsNodes = 256;
 // geometries one mesh may hold (lp_set_geometries)
 constexpr int kMaxGeoms = 64;
 

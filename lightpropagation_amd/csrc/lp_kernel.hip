@@ -739,15 +739,27 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     f3 xsft = mk(0.0f, 0.0f, 0.0f);
     int32_t adj_det = 0;  // adjoint: the detector and R weight of the replayed photon
     float adj_w = 0.0f;
-    // adjoint: one move of length s from p0 in the current tissue
-    auto adj_step = [&](f3 p0, float s, bool collision) {
+    // adjoint: one move of length s from p0 in the current tissue, ending in
+    // an interaction (collision) or at a face
+    auto adj_step = [&](f3 p0, float s, bool collision, float mus) {
         adjL[ph.curr * kBlock + tid] += s;
         if (collision) adjN[ph.curr * kBlock + tid] += 1u;
         if (a.jac) {
             const int64_t nvox = (int64_t)a.jac_n[0] * a.jac_n[1] * a.jac_n[2];
             const f3 d0 = sub(p0, POI), d1 = sub(ph.p, POI);
-            jac_dda(a, a.jac + adj_det * nvox, mk(dot(eX, d0), dot(eY, d0), dot(eZ, d0)),
-                    mk(dot(eX, d1), dot(eY, d1), dot(eZ, d1)), s, adj_w);
+            const f3 c1 = mk(dot(eX, d1), dot(eY, d1), dot(eZ, d1));
+            jac_dda(a, a.jac + adj_det * nvox, mk(dot(eX, d0), dot(eY, d0), dot(eZ, d0)), c1, s, adj_w);
+            // the interaction's mus term goes to the voxel holding it
+            if (collision && a.jac_coll && mus != 0.0f) {
+                const float gx = (c1.x - a.jac_origin[0]) * a.jac_inv[0];
+                const float gy = (c1.y - a.jac_origin[1]) * a.jac_inv[1];
+                const float gz = (c1.z - a.jac_origin[2]) * a.jac_inv[2];
+                if (gx >= 0.0f && gy >= 0.0f && gz >= 0.0f && gx < (float)a.jac_n[0] && gy < (float)a.jac_n[1] &&
+                    gz < (float)a.jac_n[2]) {
+                    const int64_t v = ((int64_t)(int32_t)gz * a.jac_n[1] + (int32_t)gy) * a.jac_n[0] + (int32_t)gx;
+                    atomic_add_u64(&a.jac_coll[adj_det * nvox + v], fx32(adj_w / mus));
+                }
+            }
         }
     };
 
@@ -756,7 +768,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         const float4 cm = lmat[ph.inst * ntp1 + ph.curr];
         const f3 p_old = ph.p;
         ph.p = add(ph.p, mul(s, ph.v));
-        if constexpr (ADJ) adj_step(p_old, s, true);
+        if constexpr (ADJ) adj_step(p_old, s, true, cm.z);
         ph.ref = -1;
         // Absorb (.cl:841-847); skipped only in a start tissue with mua or
         // mus 0, which a photon never re-enters and where nothing was absorbed
@@ -847,7 +859,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         int kind;
         const f3 p_old = ph.p;
         ph.p = add(ph.p, mul(stb, ph.v));  // MoveStep
-        if constexpr (ADJ) adj_step(p_old, stb, false);
+        if constexpr (ADJ) adj_step(p_old, stb, false, 0.0f);
         ph.ref = -1;
         ph.safe = 0.0f;
         if (h.id < 0) {  // nothing ahead: the photon has left the mesh

@@ -179,12 +179,19 @@ void lpm_mf_default(lpm_mf_params* p)
     p->torso_center[0] = 0.0;
     p->torso_center[1] = 0.0;
     p->torso_center[2] = 18.0;
+    p->regions = 10;
+    p->t_csf = 0.2;
+    p->t_grey = 0.4;
+    p->freq_fetal = 27;
+    p->freq_torso = 20;
 }
 
 int64_t lpm_mf_num_triangles(const lpm_mf_params* p)
 {
     if (!p || p->freq_maternal < 1 || p->freq_fetal < 1 || p->freq_torso < 0) return -1;
-    return 4 * 20LL * p->freq_maternal * p->freq_maternal + 3 * 20LL * p->freq_fetal * p->freq_fetal +
+    if (p->regions != 7 && p->regions != 10) return -1;
+    const int64_t head = p->regions == 10 ? 5 : 3;  // fetal spheres
+    return 4 * 20LL * p->freq_maternal * p->freq_maternal + head * 20LL * p->freq_fetal * p->freq_fetal +
            20LL * p->freq_torso * p->freq_torso;
 }
 
@@ -197,7 +204,10 @@ int lpm_mf_build(const lpm_mf_params* p, float* tri, float* nrm, int32_t* med, i
     const V3 cf = {0.0, 0.0, p->fetal_depth + p->r_head};
     // the fetal head must sit inside the amniotic cavity
     double dz = std::fabs(cf[2] - cm[2]);
-    if (!(r_cav > 0 && p->r_head > p->t_scalp + p->t_skull && dz + p->r_head < r_cav)) return 3;
+    const bool ten = p->regions == 10;
+    const double inner = p->t_scalp + p->t_skull + (ten ? p->t_csf + p->t_grey : 0.0);
+    if (!(r_cav > 0 && p->r_head > inner && dz + p->r_head < r_cav)) return 3;
+    if (ten && !(p->t_csf > 0 && p->t_grey > 0)) return 3;
     Geodesic gm = make_geodesic(p->freq_maternal);
     Geodesic gf = (p->freq_fetal == p->freq_maternal) ? gm : make_geodesic(p->freq_fetal);
     int64_t k = 0;
@@ -208,6 +218,10 @@ int lpm_mf_build(const lpm_mf_params* p, float* tri, float* nrm, int32_t* med, i
     k = emit_sphere(gf, cf, p->r_head, 5, 4, tri, nrm, med, k);
     k = emit_sphere(gf, cf, p->r_head - p->t_scalp, 6, 5, tri, nrm, med, k);
     k = emit_sphere(gf, cf, p->r_head - p->t_scalp - p->t_skull, 7, 6, tri, nrm, med, k);
+    if (ten) {
+        k = emit_sphere(gf, cf, p->r_head - p->t_scalp - p->t_skull - p->t_csf, 8, 7, tri, nrm, med, k);
+        k = emit_sphere(gf, cf, p->r_head - inner, 9, 8, tri, nrm, med, k);
+    }
     if (p->freq_torso > 0) {
         // the torso must lie in the cavity and clear of the head (checked
         // with its largest semi-axis, a bounding sphere)
@@ -222,7 +236,7 @@ int lpm_mf_build(const lpm_mf_params* p, float* tri, float* nrm, int32_t* med, i
             V3 q[3];
             for (int v = 0; v < 3; ++v)
                 for (int x = 0; x < 3; ++x) q[v][x] = ct[x] + p->torso_axes[x] * gt.pts[F[v]][x];
-            put_tri(tri, nrm, med, k++, q[0], q[1], q[2], 5, 4);
+            put_tri(tri, nrm, med, k++, q[0], q[1], q[2], ten ? 10 : 5, 4);
         }
     }
     return k == ntri ? 0 : 1;
@@ -240,7 +254,7 @@ int lpm_mf_build(const lpm_mf_params* p, float* tri, float* nrm, int32_t* med, i
 // confine every photon to the outermost one.
 int lpm_mf_optics(double nm, double spo2_m, double spo2_f, float* out, int32_t ntissue)
 {
-    if (!out || ntissue != 8 || spo2_m < 0 || spo2_m > 1 || spo2_f < 0 || spo2_f > 1) return 3;
+    if (!out || (ntissue != 8 && ntissue != 11) || spo2_m < 0 || spo2_m > 1 || spo2_f < 0 || spo2_f > 1) return 3;
     const HbRow* row = nullptr;
     for (const HbRow& r : kHb)
         if (std::fabs(r.nm - nm) < 1e-9) row = &r;
@@ -253,7 +267,10 @@ int lpm_mf_optics(double nm, double spo2_m, double spo2_f, float* out, int32_t n
         double n, B, W, a, b, g;
         int fetal;
     };
-    const T t[8] = {
+    // the 8-tissue table (regions 7) is rows 0-7; the 11-tissue table
+    // (regions 10) is rows 0-6, then 8-11 (the brain row 7 becomes grey
+    // matter, 9)
+    const T t8[12] = {
         {1.00, 0.00, 0.00, 0.0, 0.0, 0.0, 0},     // ambient
         {1.40, 0.02, 0.65, 46.0, 1.42, 0.90, 0},  // maternal skin
         {1.40, 0.01, 0.15, 20.0, 0.70, 0.90, 0},  // subcutaneous fat
@@ -261,9 +278,15 @@ int lpm_mf_optics(double nm, double spo2_m, double spo2_f, float* out, int32_t n
         {1.40, 0.00, 0.99, 0.1, 0.0, 0.90, 0},    // amniotic fluid (mus' 0.1/cm)
         {1.40, 0.03, 0.80, 46.0, 1.42, 0.90, 1},  // fetal scalp
         {1.40, 0.03, 0.30, 22.9, 0.72, 0.90, 1},  // fetal skull
-        {1.40, 0.04, 0.85, 24.2, 1.61, 0.90, 1},  // fetal brain
+        {1.40, 0.04, 0.85, 24.2, 1.61, 0.90, 1},  // fetal brain (regions 7)
+        {1.40, 0.00, 0.99, 0.2, 0.0, 0.90, 1},    // CSF (mus' 0.2/cm)
+        {1.40, 0.04, 0.85, 24.2, 1.61, 0.90, 1},  // grey matter
+        {1.40, 0.03, 0.75, 40.8, 3.09, 0.90, 1},  // white matter
+        {1.40, 0.04, 0.75, 18.9, 1.29, 0.90, 1},  // fetal body (generic soft tissue)
     };
-    for (int i = 0; i < 8; ++i) {
+    T t[11];
+    for (int i = 0; i < ntissue; ++i) t[i] = (ntissue == 8 || i < 7) ? t8[i] : t8[i + 1];
+    for (int i = 0; i < ntissue; ++i) {
         double S = t[i].fetal ? spo2_f : spo2_m;
         double mua = t[i].B * (S * mu_oxy + (1 - S) * mu_deoxy) + t[i].W * water;
         double musp = t[i].a * std::pow(nm / 500.0, -t[i].b);

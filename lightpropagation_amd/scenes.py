@@ -12,7 +12,8 @@ import numpy as np
 from ._capi import fp, ip, lib
 
 MF_TISSUES = ["ambient", "maternal skin", "fat", "uterine wall", "amniotic fluid", "fetal scalp",
-              "fetal skull", "fetal brain"]
+              "fetal skull", "CSF", "grey matter", "white matter", "fetal body"]
+MF_TISSUES_7 = MF_TISSUES[:7] + ["fetal brain"]  # the regions=7 layout
 MF_FETAL_ID = 5
 
 
@@ -21,7 +22,8 @@ class MFParams(C.Structure):
                 ("t_uterus", C.c_double), ("fetal_depth", C.c_double), ("r_head", C.c_double),
                 ("t_scalp", C.c_double), ("t_skull", C.c_double), ("freq_maternal", C.c_int32),
                 ("freq_fetal", C.c_int32), ("freq_torso", C.c_int32), ("torso_axes", C.c_double * 3),
-                ("torso_center", C.c_double * 3)]
+                ("torso_center", C.c_double * 3), ("regions", C.c_int32), ("t_csf", C.c_double),
+                ("t_grey", C.c_double)]
 
 
 def _chk(rc, what):
@@ -38,9 +40,12 @@ def emitter(p0, heading, distance):
     return ps, pb
 
 
-def mf_optics(wavelength_nm=660.0, spo2_maternal=0.97, spo2_fetal=0.50):
-    out = np.zeros((8, 4), np.float32)
-    _chk(lib().lpm_mf_optics(float(wavelength_nm), float(spo2_maternal), float(spo2_fetal), fp(out), 8),
+def mf_optics(wavelength_nm=660.0, spo2_maternal=0.97, spo2_fetal=0.50, regions=10):
+    """(n, mua, mus, g) of the maternal-fetal tissues: 11 rows (regions 10)
+    or 8 (regions 7)."""
+    nt = 11 if regions == 10 else 8
+    out = np.zeros((nt, 4), np.float32)
+    _chk(lib().lpm_mf_optics(float(wavelength_nm), float(spo2_maternal), float(spo2_fetal), fp(out), nt),
          "lpm_mf_optics")
     return out
 
@@ -52,6 +57,8 @@ def mf_mesh(**kw):
         if k in ("torso_axes", "torso_center"):
             v = (C.c_double * 3)(*v)
         setattr(p, k, v)
+    if "torso_center" not in kw:  # 0.5 cm below the fetal head, whatever its depth
+        p.torso_center[2] = p.fetal_depth + 2 * p.r_head + 0.5 + p.torso_axes[2]
     n = lib().lpm_mf_num_triangles(C.byref(p))
     tri = np.zeros((n, 9), np.float32)
     nrm = np.zeros((n, 3), np.float32)
@@ -61,12 +68,17 @@ def mf_mesh(**kw):
 
 
 def maternal_fetal(wavelength_nm=660.0, spo2_maternal=0.97, spo2_fetal=0.50, chance=0.1, w_th=1e-4, **mesh_kw):
-    """Maternal-fetal abdomen (about 2e5 faces by default), pencil beam at
-    the origin heading +z into the body."""
+    """Maternal-fetal abdomen (about 2e5 faces by default: 10 regions),
+    pencil beam at the origin heading +z into the body.  regions=7 gives
+    the earlier layout (skin, fat, uterus, fluid, scalp, skull, brain; seven
+    spheres of frequency 38 unless given otherwise, 202 160 faces)."""
+    if mesh_kw.get("regions", 10) == 7:
+        mesh_kw.setdefault("freq_fetal", 38)
+        mesh_kw.setdefault("freq_torso", 0)
     tri, nrm, med, p = mf_mesh(**mesh_kw)
     ps, pb = emitter([0, 0, 0], [0, 0, 1], 0.002)
     D = 2 * p.r_maternal
-    return dict(tri=tri, nrm=nrm, med=med, nmuasg=mf_optics(wavelength_nm, spo2_maternal, spo2_fetal),
+    return dict(tri=tri, nrm=nrm, med=med, nmuasg=mf_optics(wavelength_nm, spo2_maternal, spo2_fetal, p.regions),
                 fetal_id=MF_FETAL_ID, pstart=ps, vstart=np.array([[0, 0, 1]], np.float32),
                 vocc=np.array([1], np.int32), poi_basis=pb, model_dims=np.array([D, D, D], np.float32),
                 det_grid=np.array([0.1, 0.1, 0.1, 0.05, 1.0, 1.0], np.float32), factor=1.0, chance=chance,

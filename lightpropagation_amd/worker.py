@@ -74,7 +74,8 @@ class Server:
 
     def optics(self, wavelength_nm, spo2_maternal, spo2_fetal):
         from lightpropagation_amd import scenes
-        return scenes.mf_optics(wavelength_nm, spo2_maternal, spo2_fetal)
+        return scenes.mf_optics(wavelength_nm, spo2_maternal, spo2_fetal,
+                                regions=10 if len(self.sc["nmuasg"]) == 11 else 7)
 
     def materials(self, nmuasg):
         self.ctx.set_materials(np.asarray(nmuasg, np.float32), self.sc["fetal_id"])

@@ -577,7 +577,8 @@ static void record_exit(const orc_params *P, const orc_layout *L, orc_tallies *T
 typedef struct {
     float *L;       /* path length per tissue, accumulated in step order */
     int32_t *n;     /* interactions per tissue */
-    float *seg;     /* 7 floats per move: start (3), end (3) in the emitter frame, length */
+    float *seg;     /* 8 floats per move: start (3), end (3) in the emitter frame, length,
+                       and the tissue of the interaction that ends it (-1: a face) */
     int64_t nseg, cap;
 } adj_track;
 
@@ -587,10 +588,11 @@ static void adj_move(adj_track *A, int32_t tissue, of3 c0, of3 c1, float s, int 
     if (collision) A->n[tissue]++;
     if (A->nseg == A->cap) {
         A->cap = A->cap ? 2 * A->cap : 1024;
-        A->seg = (float *)realloc(A->seg, (size_t)A->cap * 7 * sizeof(float));
+        A->seg = (float *)realloc(A->seg, (size_t)A->cap * 8 * sizeof(float));
     }
-    float *q = A->seg + 7 * A->nseg++;
+    float *q = A->seg + 8 * A->nseg++;
     q[0] = c0.x; q[1] = c0.y; q[2] = c0.z; q[3] = c1.x; q[4] = c1.y; q[5] = c1.z; q[6] = s;
+    q[7] = collision ? (float)tissue : -1.0f;
 }
 
 /* Adds -(length in voxel) * W of the segment c0 -> c1 (length s) to each
@@ -643,6 +645,19 @@ static void dda_add(const orc_adjoint_out *A, int64_t *jac_d, const float *q, fl
         if (i[a] < 0 || i[a] >= n[a]) break;
         tmax[a] += tdel[a];
     }
+}
+
+/* The interaction term of the per-voxel mus gradient: w = W / mus added to
+ * the voxel holding the interaction point c (emitter frame), if any. */
+static void coll_add(const orc_adjoint_out *A, int64_t *jac_d, const float *c, float w)
+{
+    const int32_t *n = A->jac_n;
+    float g[3];
+    for (int a = 0; a < 3; ++a) {
+        g[a] = (c[a] - A->jac_origin[a]) * (1.0f / A->jac_size[a]);
+        if (!(g[a] >= 0.0f && g[a] < (float)n[a])) return;
+    }
+    add64(&jac_d[((int64_t)(int32_t)g[2] * n[1] + (int32_t)g[1]) * n[0] + (int32_t)g[0]], fx32(w));
 }
 
 /* TransmitOrReflect (.cl:921-1018) with GetCosAlphaCrit (:909-919),
@@ -965,7 +980,14 @@ int orc_adjoint(const orc_params *P, int64_t first, int64_t count, orc_tallies *
                         add64(&out->grad[((int64_t)d * nt + t) * 2 + 1], fx32(gs));
                     }
                 if (out->jac && nvox > 0)
-                    for (int64_t k = 0; k < A.nseg; ++k) dda_add(out, out->jac + d * nvox, A.seg + 7 * k, W);
+                    for (int64_t k = 0; k < A.nseg; ++k) dda_add(out, out->jac + d * nvox, A.seg + 8 * k, W);
+                if (out->jac_mus && nvox > 0)
+                    for (int64_t k = 0; k < A.nseg; ++k) {
+                        const float *q = A.seg + 8 * k;
+                        dda_add(out, out->jac_mus + d * nvox, q, W);
+                        const float mus = q[7] >= 0.0f ? P->nmuasg[4 * (int32_t)q[7] + 2] : 0.0f;
+                        if (mus != 0.0f) coll_add(out, out->jac_mus + d * nvox, q + 3, W / mus);
+                    }
             }
         }
 #ifdef _OPENMP

@@ -186,6 +186,10 @@ typedef struct orc_adjoint_out {
     int64_t *grad;     /* ndet * ntissue * 2: (d/dmua, d/dmus) of tissue t */
     int64_t *jac;      /* ndet * jac_n[0] * jac_n[1] * jac_n[2], x fastest */
     int64_t ndetected; /* (photon, detector) pairs, accumulated */
+    /* per-voxel mus Jacobian, ndet * nvox (NULL: none): dW/dmus_v =
+     * (sum over the interactions in v of 1 / mus) W - L_v W, accumulated as
+     * the -L_v W of jac plus W / mus at each interaction's voxel */
+    int64_t *jac_mus;
 } orc_adjoint_out;
 
 int orc_adjoint(const orc_params *p, int64_t first, int64_t count, orc_tallies *t, orc_adjoint_out *out,

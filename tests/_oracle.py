@@ -54,7 +54,7 @@ _lib = None
 
 class AdjointOut(C.Structure):
     _fields_ = [("jac_origin", C.c_float * 3), ("jac_size", C.c_float * 3), ("jac_n", C.c_int32 * 3),
-                ("signals", _lp), ("grad", _lp), ("jac", _lp), ("ndetected", C.c_int64)]
+                ("signals", _lp), ("grad", _lp), ("jac", _lp), ("ndetected", C.c_int64), ("jac_mus", _lp)]
 
 
 def lib():
@@ -287,10 +287,11 @@ class Run:
         assert rc == 0
         return ex, tr, tn
 
-    def adjoint(self, first, count, jac=None, nthreads=1, tallies=False):
+    def adjoint(self, first, count, jac=None, nthreads=1, tallies=False, jac_mus=False):
         """orc_adjoint: (signals (ndet,), grad (ndet, ntissue, 2), jac
         (ndet, nvox) or None, pairs), 2^32 fixed point; jac = (origin,
-        size, n) of the voxel grid or None."""
+        size, n) of the voxel grid or None.  With jac_mus, the per-voxel mus
+        Jacobian (ndet, nvox) is appended as a fifth element."""
         nd = self.P.ndet
         nt = self.P.ntissue
         A = AdjointOut()
@@ -305,11 +306,16 @@ class Run:
             J = np.zeros(max(nd, 1) * int(np.prod(n)), np.int64)
         A.signals, A.grad = lptr(sig), lptr(grad)
         A.jac = lptr(J) if J is not None else None
+        Js = np.zeros_like(J) if (jac_mus and J is not None) else None
+        A.jac_mus = lptr(Js) if Js is not None else None
         rc = lib().orc_adjoint(C.byref(self.P), first, count, C.byref(self.T) if tallies else None, C.byref(A),
                                nthreads)
         assert rc == 0
-        return (sig[:nd], grad[:nd * nt * 2].reshape(nd, nt, 2),
-                J.reshape(max(nd, 1), -1)[:nd] if J is not None else None, A.ndetected)
+        out = (sig[:nd], grad[:nd * nt * 2].reshape(nd, nt, 2),
+               J.reshape(max(nd, 1), -1)[:nd] if J is not None else None, A.ndetected)
+        if jac_mus:
+            out += (Js.reshape(max(nd, 1), -1)[:nd] if Js is not None else None,)
+        return out
 
     def tallies(self):
         out = {k: v.copy() for k, v in self.t.items()}

@@ -27,10 +27,10 @@ PHOTONS = [1327043, 6059547]
 
 
 def main():
-    sc = scenes.maternal_fetal()
+    sc = scenes.maternal_fetal(regions=7)
     lo, hi = bench.detectors()
     run = O.Run(sc, seed=SEED, tallies={"det_rlo": lo, "det_rhi": hi})
-    out = {"seed": SEED, "scene": "scenes.maternal_fetal() defaults, bench.setup tallies", "exit_bits": {}}
+    out = {"seed": SEED, "scene": "scenes.maternal_fetal(regions=7), bench.setup tallies", "exit_bits": {}}
     for pid in PHOTONS:
         ex, _, _ = run.run(pid, 1, exit_rec=True)
         out["exit_bits"][str(pid)] = [int(x) for x in ex[0].astype(np.float32).view(np.uint32)]

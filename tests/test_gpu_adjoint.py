@@ -34,7 +34,7 @@ def _scene(name):
         return scenes.slab(scenes.SLAB10_LAYERS), [0.0, 0.3, 1.0], [0.3, 1.0, 3.0], \
             ([-5.0, -5.0, -0.01], [0.25, 0.25, 0.02], [40, 40, 52])
     from lightpropagation_amd import scenes as S
-    return S.maternal_fetal(freq_maternal=6, freq_fetal=6), [0.0, 1.0, 3.0], [1.0, 3.0, 10.0], \
+    return S.maternal_fetal(freq_maternal=6, freq_fetal=6, freq_torso=4), [0.0, 1.0, 3.0], [1.0, 3.0, 10.0], \
         ([-6.4, -6.4, 0.0], [0.2, 0.2, 0.2], [64, 64, 64])
 
 
@@ -46,15 +46,17 @@ def test_adjoint_bit_exact_vs_oracle(name, n):
     ctx.set_scene(sc, seed=21)
     ctx.set_detectors(lo, hi)
     ctx.set_jacobian_grid(*grid)
-    got = ctx.adjoint(0, n, jac=True)
+    got = ctx.adjoint(0, n, jac=True, jac_mus=True)
     st = ctx.stats()
     ctx.close()
     run = O.Run(sc, seed=21, tallies={"det_rlo": lo, "det_rhi": hi})
-    sig, grad, jac, pairs = run.adjoint(0, n, jac=grid, nthreads=8)
+    sig, grad, jac, pairs, jac_s = run.adjoint(0, n, jac=grid, nthreads=8, jac_mus=True)
     assert st.replay_mismatch == 0 and st.replayed == pairs == got["pairs"] > 0
     assert np.array_equal(got["signals"], sig)
     assert np.array_equal(got["grad"], grad)
     assert np.array_equal(got["jac"].reshape(len(lo), -1), jac)
+    assert np.array_equal(got["jac_mus"].reshape(len(lo), -1), jac_s)
+    assert np.any(jac_s != jac)
 
 
 def _bench_ctx():
@@ -167,3 +169,88 @@ def test_adjoint_refuses_a_pixel_per_unit_factor():
     with pytest.raises(lp.LPError, match="factor"):
         ctx.adjoint(0, 100)
     ctx.close()
+
+
+def _voxel_inclusion_scene():
+    """Two layers (0.5 cm, mus 100 and 80, g 0.9, one index) with an
+    inclusion of its own tissue (3: mua 0.3, mus 10, g 0, so that its mus
+    matters enough for differences to resolve) that coincides with one voxel
+    of the Jacobian grid: the box [0.4, 0.6] x [0, 0.2] x [0.2, 0.4] in the
+    emitter frame (the grid's frame), mapped to the world by the emitter
+    basis."""
+    from lightpropagation_amd import scenes as S
+    layers = [(0.5, 1.40, 0.3, 100.0, 0.9), (2.0, 1.40, 0.2, 80.0, 0.9)]
+    sc = S.slab(layers, half=5.0)
+    pb = np.asarray(sc["poi_basis"], np.float64)
+    poi, E = pb[:3], pb[3:].reshape(3, 3)
+    lo, hi = np.array([0.4, 0.0, 0.2]), np.array([0.6, 0.2, 0.4])
+    corner = lambda i, j, k: poi + E.T @ np.array([(lo, hi)[i][0], (lo, hi)[j][1], (lo, hi)[k][2]])
+    quads = [((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0)), ((0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)),
+             ((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)), ((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0)),
+             ((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0)), ((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1))]
+    tri = []
+    centre = poi + E.T @ ((lo + hi) / 2)
+    for q in quads:
+        a, b, c, d = (corner(*v) for v in q)
+        for t in ((a, b, c), (a, c, d)):
+            n = np.cross(t[1] - t[0], t[2] - t[0])
+            if np.dot(n, np.mean(t, axis=0) - centre) < 0:  # wind outward
+                t = (t[0], t[2], t[1])
+            tri.append(np.concatenate(t))
+    tri = np.array(tri, np.float32)
+    n = np.cross(tri[:, 3:6].astype(np.float64) - tri[:, 0:3], tri[:, 6:9].astype(np.float64) - tri[:, 0:3])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    sc["tri"] = np.ascontiguousarray(np.concatenate([sc["tri"], tri]))
+    sc["nrm"] = np.ascontiguousarray(np.concatenate([sc["nrm"], n.astype(np.float32)]))
+    sc["med"] = np.ascontiguousarray(np.concatenate([sc["med"], np.tile(np.array([[3, 1]], np.int32), (12, 1))]))
+    sc["nmuasg"] = np.concatenate([sc["nmuasg"], [[1.40, 0.3, 10.0, 0.0]]]).astype(np.float32)
+    grid = ([-1.0, -1.0, 0.0], [0.2, 0.2, 0.2], [10, 10, 5])
+    voxel = (1 * 10 + 5) * 10 + 7  # iz 1, iy 5, ix 7
+    return sc, grid, voxel
+
+
+def test_voxel_jacobians_against_differences_on_one_voxel():
+    # the mua and mus Jacobians at the inclusion's voxel against central
+    # differences of the detector signals in the inclusion's mua and mus,
+    # with common random numbers (the voxel's optics are the inclusion's)
+    sc, grid, voxel = _voxel_inclusion_scene()
+    lo, hi = [0.2, 0.6, 1.2], [0.6, 1.2, 2.5]
+    ctx = lp.Context(0)
+    ctx.set_scene(sc, seed=0x5EED0021)
+    ctx.set_detectors(lo, hi)
+    ctx.set_jacobian_grid(*grid)
+    ctx.set_tallies(1 << lp.TALLY["det"] | 1 << lp.TALLY["misc"])
+    B, N, rel = 8, 4_000_000, 0.05
+    adj = np.zeros((B, 2, 3))
+    tis = np.zeros((B, 2, 3))
+    for b in range(B):
+        r = ctx.adjoint(b * N, N, jac=True, jac_mus=True)
+        adj[b, 0] = r["jac"].reshape(3, -1)[:, voxel] / FX / N
+        adj[b, 1] = r["jac_mus"].reshape(3, -1)[:, voxel] / FX / N
+        tis[b] = r["grad"][:, 3, :].T / FX / N  # the inclusion tissue's own gradients
+    assert ctx.stats().replay_mismatch == 0
+    nm0 = np.asarray(sc["nmuasg"], np.float32)
+    fd = np.zeros_like(adj)
+    for i, c in enumerate((1, 2)):
+        h = float(nm0[3, c]) * rel
+        S = {}
+        for sgn in (-1, 1):
+            nm = nm0.copy()
+            nm[3, c] = np.float32(nm0[3, c] + sgn * h)
+            ctx.set_materials(nm, -1)
+            S[sgn] = []
+            for b in range(B):
+                ctx.reset()
+                ctx.run(b * N, N)
+                S[sgn].append(ctx.tallies()["det"].reshape(-1, 2)[:, 0] / FX / N)
+        dh = float(np.float32(nm0[3, c] + h)) - float(np.float32(nm0[3, c] - h))
+        for b in range(B):
+            fd[b, i] = (S[1][b] - S[-1][b]) / dh
+    ctx.close()
+    a, f = adj.mean(0), fd.mean(0)
+    se = np.sqrt(adj.var(0, ddof=1) / B + fd.var(0, ddof=1) / B)
+    print("voxel adjoint", a, "\nfinite differences", f, "\nse", se, flush=True)
+    # the voxel holds the inclusion: its Jacobian is the inclusion's gradient
+    assert np.allclose(a, tis.mean(0), rtol=2e-3, atol=1e-9)
+    assert np.all(np.abs(a) > 3 * se)  # resolved
+    assert np.all(np.abs(a - f) <= 4 * se + 0.05 * np.abs(a)), (a, f, se)

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _ctx(seed=31):
-    sc = S.maternal_fetal(freq_maternal=8, freq_fetal=8)
+    sc = S.maternal_fetal(freq_maternal=8, freq_fetal=8, freq_torso=4)
     ctx = lp.Context(0)
     ctx.set_scene(sc, seed=seed)
     c = 2.0 + 8.0 * np.arange(8) / 7.0
@@ -120,7 +120,7 @@ def test_grid_follows_occupancy():
 
 def test_batch_rejects_oversized_tables():
     ctx, sc = _ctx()
-    tab = np.repeat(_table([(660.0, 0.97, 0.5)]), 200, axis=0)  # 200 x 9 rows > 1152
+    tab = np.repeat(_table([(660.0, 0.97, 0.5)]), 200, axis=0)  # 200 x 12 rows > 2048
     with pytest.raises(lp.LPError):
         ctx.run_batch(tab, 0, 10)
     ctx.close()

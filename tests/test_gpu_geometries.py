@@ -25,7 +25,7 @@ TALLIES = 1 << lp.TALLY["det"] | 1 << lp.TALLY["region"] | 1 << lp.TALLY["misc"]
 
 
 def _scenes():
-    return [S.maternal_fetal(fetal_depth=d, freq_maternal=f, freq_fetal=f, **kw) for d, f, kw in GEOMS]
+    return [S.maternal_fetal(fetal_depth=d, freq_maternal=f, freq_fetal=f, freq_torso=4, **kw) for d, f, kw in GEOMS]
 
 
 def _ctx(sc, seed, counts=None, near=True):

@@ -40,7 +40,7 @@ def _scene(name):
         return scenes.slab(scenes.C0_LAYERS)
     if name == "mf_coarse":  # maternal-fetal abdomen, 5 040 faces
         from lightpropagation_amd import scenes as S
-        return S.maternal_fetal(freq_maternal=6, freq_fetal=6)
+        return S.maternal_fetal(freq_maternal=6, freq_fetal=6, freq_torso=4)
     if name == "model6FB":
         return scenes.load_fixture("model6", "configFB.xml")
     if name == "planar":  # model4/configPlanar.xml: 20 planar faces, 160 x 160 cm
@@ -181,9 +181,10 @@ def test_bench_scene_photons_that_turned_nan():
     # face at |cos| = 1.00000012 and its direction became NaN in the
     # reference's Transmit (sqrt of 1 - cos^2 < 0); NaN rays then opened
     # every BVH box and the launch ran for minutes.  The range around it must
-    # now finish quickly with no NaN photon and no capped photon.
+    # now finish quickly with no NaN photon and no capped photon.  (The
+    # seven-region mesh of that workload.)
     from lightpropagation_amd import scenes as S
-    sc = S.maternal_fetal()
+    sc = S.maternal_fetal(regions=7)
     ctx = lp.Context(0)
     ctx.set_scene(sc, seed=0x5EED0001, max_steps=200000)
     ctx.set_tallies(record_exit=True)
@@ -255,7 +256,7 @@ def test_face_met_at_exactly_the_step_length():
     from lightpropagation_amd import scenes as S
     import bench
     g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "tie_photons.json")))
-    sc = S.maternal_fetal()
+    sc = S.maternal_fetal(regions=7)  # the seven-region mesh the ties were found on
     for opts in ({"near_r0": 0.0}, {}):
         ctx = lp.Context(0)
         for k, v in opts.items():

@@ -27,46 +27,61 @@ def _closed(tri_sub):
     return all(n == 1 and edges.get((e[1], e[0]), 0) == 1 for e, n in edges.items())
 
 
+@pytest.mark.parametrize("regions", [10, 7])
 @pytest.mark.parametrize("torso", [0, 6])
-def test_mesh_counts_tags_closure_and_normals(torso):
-    tri, nrm, med, p = S.mf_mesh(freq_maternal=5, freq_fetal=4, freq_torso=torso)
-    assert len(tri) == 4 * 20 * 25 + 3 * 20 * 16 + 20 * torso * torso
+def test_mesh_counts_tags_closure_and_normals(torso, regions):
+    tri, nrm, med, p = S.mf_mesh(freq_maternal=5, freq_fetal=4, freq_torso=torso, regions=regions)
+    head = 5 if regions == 10 else 3
+    assert len(tri) == 4 * 20 * 25 + head * 20 * 16 + 20 * torso * torso
     surf = _surfaces(tri, med)
     want = {(1, 0), (2, 1), (3, 2), (4, 3), (5, 4), (6, 5), (7, 6)}
+    if regions == 10:
+        want |= {(8, 7), (9, 8)} | ({(10, 4)} if torso else set())
     assert set(surf) == want
+    torso_tag = (10, 4) if regions == 10 else (5, 4)
     cm = np.array([0.0, 0.0, p.r_maternal])
     ch = np.array([0.0, 0.0, p.fetal_depth + p.r_head])
+    tc = np.array(p.torso_center[:])
     for tag, ks in surf.items():
         f = tri[ks].reshape(-1, 3, 3).astype(np.float64)
-        if tag == (5, 4) and torso:
+        c = f.mean(1)
+        if tag == (5, 4) and torso and regions == 7:
             # the head's outer sphere and the torso share the tag: split them
-            head = np.linalg.norm(f.mean(1) - ch, axis=1) < p.r_head + 0.5
-            parts = [f[head], f[~head]]
+            in_head = np.linalg.norm(c - ch, axis=1) < p.r_head + 0.5
+            parts = [f[in_head], f[~in_head]]
+            centre = np.where(in_head[:, None], ch, tc)
         else:
             parts = [f]
+            centre = tc if tag == torso_tag and torso and regions == 10 else (cm if tag[0] <= 4 else ch)
         for part in parts:
             assert _closed(part.reshape(len(part), 9)), tag
         # normals point away from the centre of the shape they bound (outward)
-        c = f.mean(1)
-        centre = cm if tag[0] <= 4 else ch
         n = nrm[ks].astype(np.float64)
-        if tag == (5, 4) and torso:
-            tc = np.array(p.torso_center[:])
-            centre = np.where(np.linalg.norm(c - ch, axis=1, keepdims=True) < p.r_head + 0.5, ch, tc)
         assert np.all(np.einsum("ij,ij->i", n, c - centre) > 0), tag
         assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-6)
+    if regions == 10:  # the head's shells at the stated radii
+        r = [p.r_head, p.r_head - p.t_scalp, p.r_head - p.t_scalp - p.t_skull,
+             p.r_head - p.t_scalp - p.t_skull - p.t_csf, p.r_head - p.t_scalp - p.t_skull - p.t_csf - p.t_grey]
+        for tag, rr in zip([(5, 4), (6, 5), (7, 6), (8, 7), (9, 8)], r):
+            ks = [k for k in surf[tag] if np.linalg.norm(tri[k, :3] - ch) < p.r_head + 0.5]
+            d = np.linalg.norm(tri[ks].reshape(-1, 3).astype(np.float64) - ch, axis=1)
+            assert np.allclose(d, rr, atol=1e-5), tag
 
 
 def test_torso_lies_in_the_cavity_and_clear_of_the_head():
     tri, nrm, med, p = S.mf_mesh(freq_maternal=4, freq_fetal=4, freq_torso=12)
     f = tri[-20 * 144:].reshape(-1, 3).astype(np.float64)
-    assert np.all(med[-20 * 144:] == [5, 4])
+    assert np.all(med[-20 * 144:] == [10, 4])  # the fetal body, its own region
     r_cav = p.r_maternal - p.t_skin - p.t_fat - p.t_uterus
     assert np.all(np.linalg.norm(f - [0, 0, p.r_maternal], axis=1) < r_cav)
     assert np.all(np.linalg.norm(f - [0, 0, p.fetal_depth + p.r_head], axis=1) > p.r_head)
     # the vertices lie on the ellipsoid
     q = (f - np.array(p.torso_center[:])) / np.array(p.torso_axes[:])
     assert np.allclose(np.linalg.norm(q, axis=1), 1.0, atol=1e-5)
+    # the torso follows the head down: 0.5 cm below it at every C2 depth
+    for depth, kw in ((1.5, {"t_fat": 0.3, "t_uterus": 0.5}), (2.5, {}), (3.5, {}), (4.5, {})):
+        _, _, _, q = S.mf_mesh(freq_maternal=4, freq_fetal=4, freq_torso=4, fetal_depth=depth, **kw)
+        assert q.torso_center[2] == depth + 2 * q.r_head + 0.5 + q.torso_axes[2]
 
 
 def test_misplaced_torso_is_rejected():
@@ -77,12 +92,17 @@ def test_misplaced_torso_is_rejected():
 
 
 def test_large_mesh_size():
-    # the 2e6-face mesh: frequency 118 spheres and a frequency-40 torso
+    # the 2e6-face mesh: frequency-118 maternal spheres, frequency-92 fetal
+    # head spheres and a frequency-40 torso; the default 10-region mesh
     import ctypes as C
     p = S.MFParams()
     S.lib().lpm_mf_default(C.byref(p))
-    p.freq_maternal = p.freq_fetal = 118
-    p.freq_torso = 40
+    assert p.regions == 10
+    assert S.lib().lpm_mf_num_triangles(C.byref(p)) == 4 * 20 * 38 ** 2 + 5 * 20 * 27 ** 2 + 20 * 20 ** 2 == 196_420
+    p.freq_maternal, p.freq_fetal, p.freq_torso = 118, 92, 40
+    n = 4 * 20 * 118 * 118 + 5 * 20 * 92 * 92 + 20 * 40 * 40
+    assert S.lib().lpm_mf_num_triangles(C.byref(p)) == n == 1_992_320
+    p.regions, p.freq_fetal = 7, 118  # the seven-region 2e6-face mesh
     assert S.lib().lpm_mf_num_triangles(C.byref(p)) == 7 * 20 * 118 * 118 + 20 * 40 * 40 == 1_981_360
 
 

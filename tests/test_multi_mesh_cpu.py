@@ -71,7 +71,8 @@ def _walk(nodes, root):
 
 
 def _scenes():
-    return [S.maternal_fetal(fetal_depth=d, freq_maternal=f, freq_fetal=f) for d, f in ((2.5, 4), (3.5, 6), (3.0, 5))]
+    return [S.maternal_fetal(fetal_depth=d, freq_maternal=f, freq_fetal=f, freq_torso=3)
+            for d, f in ((2.5, 4), (3.5, 6), (3.0, 5))]
 
 
 @pytest.mark.parametrize("lds", [64, 1000])

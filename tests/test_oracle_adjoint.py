@@ -133,3 +133,55 @@ def test_oracle_adjoint_refuses_a_pixel_per_unit_factor():
     sig = np.zeros(1, np.int64)
     A.signals = O.lptr(sig)
     assert O.lib().orc_adjoint(C.byref(run.P), 0, 10, None, C.byref(A), 1) == 3
+
+
+def _coords32(pos, pb):
+    """The oracle's emitter-frame coordinates in float32: (p - POI) . e with
+    each basis vector normalised as p * (1 / sqrt(p . p))."""
+    f = np.float32
+    d = (np.asarray(pos, f) - np.asarray(pb[:3], f)).astype(f)
+    out = []
+    for k in range(3):
+        e = np.asarray(pb[3 + 3 * k:6 + 3 * k], f)
+        inv = f(1.0) / np.sqrt(f((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2]))
+        e = (inv * e).astype(f)
+        out.append(f(f(e[0] * d[0] + e[1] * d[1]) + f(e[2] * d[2])))
+    return np.array(out, f)
+
+
+def test_voxel_mus_gradient_on_traces():
+    # dS/dmus_v = (sum over the interactions in v of 1 / mus) W - L_v W: the
+    # difference from the mua Jacobian is W / mus booked at each detected
+    # path's interaction points, recomputed here from the step traces
+    sc, run = _c0_run(seed=13)
+    grid = ([-5.0, -5.0, -0.01], [0.25, 0.25, 0.0205], [40, 40, 155])
+    N = 500
+    sig, grad, jac, pairs, jac_s = run.adjoint(0, N, jac=grid, jac_mus=True)
+    ex, tr, tn = run.run(0, N, exit_rec=True, trace_max=20000)
+    lo, hi = [0.0, 0.3, 1.0], [0.3, 1.0, 3.0]
+    o, sz, n = (np.asarray(x, np.float32) for x in grid)
+    inv = (np.float32(1.0) / sz).astype(np.float32)
+    want = np.zeros_like(jac_s)
+    pb = np.asarray(sc["poi_basis"], np.float32)
+    for i in range(N):
+        W, radius = ex[i, 3], ex[i, 7]
+        if W == 0:
+            continue
+        tissue = 0
+        for k in range(tn[i]):
+            rec = tr[i, k]
+            if rec[0] == 0:  # an interaction, in the tissue the photon is in
+                g = ((_coords32(rec[4:7], pb) - o) * inv).astype(np.float32)
+                if np.all(g >= 0) and np.all(g < n):
+                    v = (int(g[2]) * int(n[1]) + int(g[1])) * int(n[0]) + int(g[0])
+                    q = int(np.float32(W / np.float32(sc["nmuasg"][tissue][2])) * np.float32(FX))
+                    for d in range(3):
+                        if lo[d] <= radius < hi[d]:
+                            want[d, v] += q
+            tissue = int(rec[2])
+    assert np.array_equal(jac_s - jac, want)
+    # summed over a grid that holds every path: the per-tissue mus gradients
+    for d in range(3):
+        tissue = grad[d, :, 1].sum() / FX
+        voxels = jac_s[d].sum() / FX
+        assert abs(voxels - tissue) <= 1e-5 * abs(tissue) + 1e-5, (d, voxels, tissue)

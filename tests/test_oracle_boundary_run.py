@@ -28,8 +28,9 @@ RUN = 10_000
 
 
 def trapped_scene():
-    sc = S.maternal_fetal(fetal_depth=1.5, freq_maternal=19, freq_fetal=19, t_fat=0.3, t_uterus=0.5)
-    sc["nmuasg"] = S.mf_optics(940.0, 0.97, float(np.linspace(0.30, 0.75, 16)[10]))
+    # the seven-region thin geometry of the batched workload it was found in
+    sc = S.maternal_fetal(regions=7, fetal_depth=1.5, freq_maternal=19, freq_fetal=19, t_fat=0.3, t_uterus=0.5)
+    sc["nmuasg"] = S.mf_optics(940.0, 0.97, float(np.linspace(0.30, 0.75, 16)[10]), regions=7)
     return sc
 
 

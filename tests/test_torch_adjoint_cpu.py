@@ -126,7 +126,8 @@ class OpticsSim:
 @pytest.mark.parametrize("wl", [660.0, 940.0])
 def test_spo2_optics_is_lpm_mf_optics(wl):
     o = TA.SpO2Optics(OpticsSim(), wl)
-    assert o.fetal.numpy().tolist() == [False] * 5 + [True] * 3
+    # fetal tissues with blood: scalp, skull, grey and white matter, body (not the CSF)
+    assert o.fetal.numpy().tolist() == [False] * 5 + [True, True, False, True, True, True]
     rng = np.random.default_rng(1)
     for sm, sf in rng.uniform(0, 1, (20, 2)):
         want = scenes.mf_optics(wl, sm, sf)
@@ -135,7 +136,9 @@ def test_spo2_optics_is_lpm_mf_optics(wl):
         assert np.array_equal(o.mus.numpy(), want[:, 2].astype(np.float64))
     # d mua / d SpO2 has the sign of (HbO2 - Hb) extinction: negative at 660 nm, positive at 940 nm
     b = o.b.numpy()[1:]
-    assert np.all(b[[0, 1, 2, 4, 5, 6]] < 0) if wl == 660.0 else np.all(b[[0, 1, 2, 4, 5, 6]] > 0)
+    blood = [0, 1, 2, 4, 5, 7, 8, 9]  # tissues 1-3, 5, 6, 8-10 (fluid and CSF hold no blood)
+    assert np.all(b[blood] < 0) if wl == 660.0 else np.all(b[blood] > 0)
+    assert np.all(b[[3, 6]] == 0)
 
 
 class BeerLambertSim(OpticsSim):
@@ -146,17 +149,17 @@ class BeerLambertSim(OpticsSim):
     def __init__(self):
         d = np.linspace(2, 10, 8)
         self.nmuasg = scenes.mf_optics(660.0, 0.97, 0.5)
-        C = np.zeros((8, 8))
+        C = np.zeros((8, len(self.nmuasg)))
         C[:, 1] = 0.8 + 0.1 * d
         C[:, 2] = 1.0 + 0.6 * d
         C[:, 3] = 0.3 * d
         C[:, 4] = 0.05 * d
-        C[:, 5:8] = np.maximum(d - 3, 0)[:, None] * np.array([0.05, 0.1, 0.3])
+        C[:, 5:11] = np.maximum(d - 3, 0)[:, None] * np.array([0.05, 0.1, 0.02, 0.2, 0.1, 0.05])
         self.C = C
 
     def adjoint(self, nm, first, count, jac=False):
         S = np.exp(-self.C @ nm[:, 1].astype(np.float64))
-        G = np.zeros((8, 8, 2))
+        G = np.zeros((8, len(nm), 2))
         G[..., 0] = -self.C * S[:, None]
         return dict(signals=S, grad=G, jac=None, pairs=1)
 

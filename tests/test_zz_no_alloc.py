@@ -71,7 +71,7 @@ def _noalloc(ctx, tab, geom, first, count, det, reg):
 
 def test_noalloc_equals_batch_on_two_calls():
     hip = _hip()
-    scs = [S.maternal_fetal(fetal_depth=d, freq_maternal=f, freq_fetal=f, **kw) for d, f, kw in GEOMS]
+    scs = [S.maternal_fetal(fetal_depth=d, freq_maternal=f, freq_fetal=f, freq_torso=4, **kw) for d, f, kw in GEOMS]
     merged, counts = S.geometries(scs)
     ctx = lp.Context(0)
     ctx.set_scene(merged, seed=0x5EED000A)
@@ -104,7 +104,7 @@ def test_noalloc_equals_batch_on_two_calls():
 
 def test_noalloc_refusals():
     hip = _hip()
-    sc = S.maternal_fetal(freq_maternal=6, freq_fetal=6)
+    sc = S.maternal_fetal(freq_maternal=6, freq_fetal=6, freq_torso=4)
     ctx = lp.Context(0)
     ctx.set_scene(sc, seed=3)
     ctx.set_detectors([2.0], [3.0])
@@ -113,7 +113,7 @@ def test_noalloc_refusals():
     rc = _noalloc(ctx, tab, [0], 0, 100, det, reg)  # the scene is not on the device yet
     assert rc == 6 and b"not on the device" in lib().lp_last_error(ctx._c)
     ctx.run(0, 1000)
-    big = np.ascontiguousarray(np.repeat(tab, 200, axis=0))  # 200 x 9 rows > 1152
+    big = np.ascontiguousarray(np.repeat(tab, 200, axis=0))  # 200 x 12 rows > 2048
     rc = _noalloc(ctx, big, np.zeros(200, np.int32), 0, 10, det, reg)
     assert rc == 3 and b"static table" in lib().lp_last_error(ctx._c)
     assert _noalloc(ctx, tab, [0], 0, 100, det, reg) == 0

@@ -54,7 +54,8 @@ extern "C" {
                              tests, skipped searches, NaN photons, free-radius
                              queries, their node visits, list face tests,
                              wavefront iterations, near-list search batches,
-                             BVH search batches (32 slots)                    */
+                             BVH search batches, replay mismatches, marches
+                             (32 slots)                                       */
 #define LP_NTALLY 9
 
 /* Tally values are int64 fixed point: 2^32 per unit photon weight, except
@@ -256,7 +257,7 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
                                    forced and finished lanes refilled at its end)  */
 #define LP_OPT_WAVES_PER_SIMD 7 /* register budget of the kernel variant: 1 (compiler's
                                    choice), 4 (default), 5 or 6 waves per SIMD     */
-#define LP_OPT_NEAR_R0 8        /* near-face lists: radius r0 (0 = off, default 0.05) */
+#define LP_OPT_NEAR_R0 8        /* near-face lists: radius r0 (0 = off, default 0.07) */
 #define LP_OPT_NEAR_CELL 9      /* near-face lists: cell edge (0 = 2 r0)            */
 #define LP_OPT_SEARCH_MIN 10    /* run a BVH search batch once this fraction of a
                                    wavefront's live lanes waits for one (0..1)     */
@@ -285,6 +286,11 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
                                    lists and the BVH still answer the searches.
                                    For tests: the exhaustive form of the face
                                    search, whose results the shortcuts must equal */
+#define LP_OPT_MARCH 18         /* a step longer than the near-face radius (or a flight)
+                                   whose face the lists do not find within the radius
+                                   marches on along its ray through the near-face grid
+                                   -- cell clearances over empty space, list queries
+                                   near faces -- (1, default) or searches the BVH (0) */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
 
 /* ---- batched launch (replaces ExecKernel's loop of 6 400-photon launches,

@@ -601,8 +601,14 @@ bool build_multi_mesh(const float* tri, const int32_t* med, const std::vector<in
     for (int g = 0; g < ng; ++g) {
         const int64_t f = first[g], n = first[g + 1] - first[g];
         part[g] = build_bvh(tri + 9 * f, med + 2 * f, n, 4);
-        for (int64_t t = f; t < first[g + 1]; ++t)
-            for (int v = 0; v < 3; ++v) all.grow(tri + 9 * t + 3 * v);
+        // the vertices as build_near_grid sees them (m, m + e, m + f in
+        // float), so that each part's box lies in the frame fitted to this one
+        for (const TriRec& t : part[g].tris)
+            for (int k = 0; k < 3; ++k) {
+                float v[3];
+                for (int a = 0; a < 3; ++a) v[a] = t.m[a] + (k == 1 ? t.e[a] : k == 2 ? t.f[a] : 0.0f);
+                all.grow(v);
+            }
         pad = std::max(pad, part[g].pad);
         out.bvh.depth = std::max(out.bvh.depth, part[g].depth);
     }

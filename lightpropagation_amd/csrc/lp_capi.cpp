@@ -63,11 +63,12 @@ struct lp_context {
     int32_t blocks_per_cu = 0;  // 0: as many as one CU holds at once (the variant's occupancy)
     int32_t free_iters = 32;
     float search_min = 1.0f;
-    float list_min = 0.25f;
+    float list_min = 0.6f;
     int32_t refill_min = 16;
     float clear_step = 0.0f;
     int32_t list_long = 1;
     int32_t no_skip = 0;
+    int32_t march = 1;
     int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
     float jac_origin[3] = {0, 0, 0}, jac_size[3] = {1, 1, 1};
@@ -78,7 +79,7 @@ struct lp_context {
     Bvh bvh;
     SafetyGrid sg;
     NearGrid ng;
-    float near_r0 = 0.05f, near_cell = 0.0f;  // cell 0 = 2 r0
+    float near_r0 = 0.07f, near_cell = 0.0f;  // cell 0 = 2 r0
     std::vector<int32_t> part_root;   // BVH root of each geometry
     std::vector<uint32_t> part_coff;  // near-grid cell offset of each geometry
     DevBuf d_near_cells, d_near_tris;
@@ -333,6 +334,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.clear_step = c->clear_step;
     a.list_long = c->list_long;
     a.no_skip = c->no_skip;
+    a.march = c->march;
     a.refill_min = c->refill_min;
     a.waves = c->waves;
     a.safe_margin = 2e-6f * c->bvh_extent;
@@ -703,6 +705,10 @@ int lp_set_option(lp_context* c, int32_t key, double v)
         if (v != 0 && v != 1) return fail(c, LP_EINVAL, "list_long is 0 or 1");
         c->list_long = (int32_t)v;
         return LP_OK;
+    case LP_OPT_MARCH:
+        if (v != 0 && v != 1) return fail(c, LP_EINVAL, "march is 0 or 1");
+        c->march = (int32_t)v;
+        return LP_OK;
     case LP_OPT_NO_SKIP:
         if (v != 0 && v != 1) return fail(c, LP_EINVAL, "no_skip is 0 or 1");
         c->no_skip = (int32_t)v;
@@ -1025,7 +1031,7 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
     std::vector<Knob> knobs = {{"waves", {1, 4, 5, 6}, &w},
                                {"free_iters", {16, 32, 64}, &fi},
                                {"refill_min", {8, 16, 24, 32}, &rm},
-                               {"list_min", {0.125, 0.25, 0.5}, &lm},
+                               {"list_min", {0.25, 0.4, 0.5, 0.6, 0.7}, &lm},
                                {"search_min", {0.5, 0.75, 1.0}, &sm}};
     auto apply = [&]() {
         c->waves = (int32_t)w;

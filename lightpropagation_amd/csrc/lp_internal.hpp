@@ -149,6 +149,8 @@ struct KArgs {
     float clear_step;      // cell-clearance free check for steps of at least this length (< 0: off)
     int32_t list_long;     // steps longer than near_r0 try the near-face lists (a face within near_r0) before the BVH
     int32_t no_skip;       // 1: search every step (no free region, clearance, safety grid or distance query)
+    int32_t march;         // 1: a long step the lists miss marches on through the grid (cell clearances and
+                           // further list queries) instead of searching the BVH
     int32_t refill_min;    // end the round once this many lanes of a wavefront have finished
     int32_t waves;         // occupancy target of the kernel variant (1, 4, 5, 6)
     float nmuasg[4 * kMaxTissues];
@@ -224,7 +226,8 @@ constexpr int kAdjMaxTissues = 16;  // tissues the adjoint replay tracks per lan
 enum MiscSlot {  // slots of the misc tally (kMiscN int64 words)
     kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
     kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscDistQueries,
-    kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds, kMiscReplayMismatch, kMiscUsed,
+    kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds, kMiscReplayMismatch, kMiscMarches,
+    kMiscUsed,
     kMiscN = 32
 };
 

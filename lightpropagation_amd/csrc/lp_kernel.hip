@@ -403,8 +403,10 @@ __device__ __forceinline__ Geom geom_of(const KArgs& a, const float4* lmat, int3
 // nearest face and tie rule of find_face) and, when want, the two smallest
 // distances from p to the listed faces with the record of the smaller;
 // unlisted = a lower bound on the distance to every face not in the list.
+// The list is that of the cell holding cq (a point on the ray: p, or where
+// a march along it has got to); rays and distances are from p.
 // Must be called by the whole wavefront.
-__device__ __forceinline__ void near_batch(const KArgs& a, bool part, uint32_t coff, f3 p, f3 v, int32_t ref,
+__device__ __forceinline__ void near_batch(const KArgs& a, bool part, uint32_t coff, f3 cq, f3 p, f3 v, int32_t ref,
                                         float tmax, bool want, Hit& h, float& d1sq, float& d2sq, int32_t& i1,
                                         float& unlisted, uint32_t& pairs)
 {
@@ -417,7 +419,7 @@ __device__ __forceinline__ void near_batch(const KArgs& a, bool part, uint32_t c
     unlisted = a.near_r0;  // the grid reaches r0 beyond every face
     uint32_t b = 0, n = 0;
     if (part) {
-        int64_t c = near_cell(a, p);
+        int64_t c = near_cell(a, cq);
         if (c >= 0) {
             c += coff;
             const CellRec cr = a.near_cells[c];
@@ -649,7 +651,7 @@ __device__ __forceinline__ void jac_dda(const KArgs& a, unsigned long long* jd, 
 // per-wavefront event counters in LDS
 enum WaveCount {
     wPhot, wInt, wBnd, wEsc, wCap, wSkip, wDq, wNan, wLst, wIter, wLRounds, wBRounds, wArzOut, wVoxOut, wMismatch,
-    kWc
+    wMarch, kWc
 };
 
 // WAVES: minimum waves per SIMD the register allocator must allow (1 =
@@ -659,8 +661,23 @@ enum WaveCount {
 // MG: a batched launch over several geometries -- each photon's searches
 // start at its instance's BVH root and near-grid cell offset.
 template <int RNG, bool TRACE, int WAVES, bool ADJ, bool MG>
-__global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
+__global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
 {
+    // The arguments are read through a pointer that each iteration makes
+    // opaque to the compiler: a field is then loaded from the argument
+    // segment (scalar cache) where it is used instead of being held in a
+    // scalar register for the whole loop, which would exceed the scalar
+    // register file and spill to vector lanes.
+    // (the argument is the whole kernel-argument segment, from offset 0)
+    typedef const __attribute__((address_space(4))) KArgs CKArgs;
+    CKArgs* ap = (CKArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    // a fresh copy of that pointer which the compiler cannot see through
+    auto launder = [&]() {
+        CKArgs* p = (CKArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+        asm volatile("" : "+s"(p));
+        ap = p;
+    };
+#define a (*(const KArgs*)ap)
     extern __shared__ float4 smem[];
     __shared__ unsigned long long wcnt[kBlock / 64][kWc];
     float4* lnodes = smem;                         // 2 * nlds
@@ -729,6 +746,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
     unsigned long long vox_acc = 0;
 
     float stepsize = 0.0f;  // drawn step length, pending until used
+    float mq = 0.0f;        // a long step's search: the length of the ray shown free of faces so far
     bool ballistic = false;
     bool have_step = false;  // a step length has been drawn and not yet used
     bool search = false;     // the pending step needs a face search
@@ -1185,6 +1203,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
         // the round; a round also ends once enough lanes have finished, so
         // that they are refilled.
         for (int it = 0;; ++it) {
+            launder();
             bool go = false;     // this lane takes a step in this iteration
             bool freed = false;  // ... and the step was shown to stay in the free region
             bool bnd = false;
@@ -1242,6 +1261,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                     wcount(wSkip);
                 } else {
                     search = true;
+                    mq = 0.0f;
                     const bool finite_p =
                         __builtin_isfinite(ph.p.x) && __builtin_isfinite(ph.p.y) && __builtin_isfinite(ph.p.z);
                     // the near-face lists answer a step of at most r0, and a
@@ -1264,25 +1284,76 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
                 const bool part = search && by_list;
                 float d1sq, d2sq, unlisted;
                 int32_t i1;
-                // a flight or a step longer than r0: the lists see only its
-                // first r0 (every face nearer than r0 is listed, so a hit
-                // there is the nearest face of the whole mesh)
+                // a flight or a step longer than r0: the lists see only r0 of
+                // it from the point the march along it has reached (every
+                // face within r0 of that point is listed and the ray before
+                // it is free of faces, so a hit there is the nearest face of
+                // the whole mesh)
                 const bool reach = ballistic || stepsize > a.near_r0;
                 const bool want = !a.no_skip && ph.ref < 0 && !reach;
+                const float tlim = ballistic ? 10000.0f : stepsize;
+                const uint32_t coff = part ? geom_of<MG>(a, lmat, ph.inst).coff : 0u;
+                f3 cq = ph.p;
+                float wmax = reach ? a.near_r0 : stepsize;
+                if (part && reach && mq > 0.0f) {
+                    cq = add(ph.p, mul(mq, ph.v));
+                    const float rq = fmaxf(fmaxf(__builtin_fabsf(cq.x), __builtin_fabsf(cq.y)), __builtin_fabsf(cq.z));
+                    wmax = fminf(mq + (a.near_r0 - (a.safe_margin + 1e-5f * (a.near_r0 + rq))), tlim);
+                }
                 Hit hl;
                 uint32_t pairs;
-                near_batch(a, part, part ? geom_of<MG>(a, lmat, ph.inst).coff : 0u, ph.p, ph.v, ph.ref,
-                           reach ? a.near_r0 : stepsize, want, hl, d1sq, d2sq, i1, unlisted, pairs);
+                near_batch(a, part, coff, cq, ph.p, ph.v, ph.ref, wmax, want, hl, d1sq, d2sq, i1, unlisted, pairs);
                 if (lane == 0) atomicAdd(&wc[wLst], (unsigned long long)pairs);
                 if (part && reach) {
-                    if (hl.id >= 0) {  // the nearest face ahead, within r0
+                    if (hl.id >= 0) {  // the nearest face ahead
                         h = hl;
                         if (ballistic) stepsize = h.t;
                         bnd = true;
                         search = false;
                         go = true;
+                    } else if (!a.march || near_cell(a, ph.p) < 0) {
+                        by_list = false;  // the BVH
                     } else {
-                        by_list = false;  // farther than r0: the BVH
+                        // March on: from each point the cell's clearance (no
+                        // face within it, less a rounding margin) frees the
+                        // ray ahead, until a face is near (the next list
+                        // query looks from there), the step ends, or the ray
+                        // leaves the grid (a box reaching beyond every face:
+                        // nothing lies ahead then).
+                        float cleared = wmax;
+                        bool exited = false;
+                        int n = 0;
+                        while (cleared < tlim && n < 256) {
+                            ++n;
+                            const f3 q = add(ph.p, mul(cleared, ph.v));
+                            const int64_t c = near_cell(a, q);
+                            if (c < 0) {
+                                exited = true;
+                                break;
+                            }
+                            const float cl = __uint_as_float(a.near_cells[c + coff].clear_bits);
+                            const float rq =
+                                fmaxf(fmaxf(__builtin_fabsf(q.x), __builtin_fabsf(q.y)), __builtin_fabsf(q.z));
+                            const float sc = cl - (a.safe_margin + 1e-5f * (cl + rq));
+                            if (!(sc > 0.0f)) break;
+                            cleared += sc;
+                        }
+                        wcount(wMarch);
+                        if (exited || cleared >= tlim) {
+                            // no face within the step: a flight (or a step of
+                            // 10^4 or more) leaves the mesh, a step ends in
+                            // its tissue
+                            h.id = -1;
+                            h.t = 10000.0f;
+                            bnd = ballistic || stepsize >= 10000.0f;
+                            if (ballistic) stepsize = h.t;
+                            search = false;
+                            go = true;
+                        } else if (n >= 256) {
+                            by_list = false;  // a march this long: the BVH
+                        } else {
+                            mq = cleared;  // a face is near: the next list query looks from here
+                        }
                     }
                 } else if (part) {
                     h = hl;
@@ -1412,8 +1483,10 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a)
             atomic_add_u64(&a.misc[kMiscArzOut], wc[wArzOut]);
             atomic_add_u64(&a.misc[kMiscVoxOut], wc[wVoxOut]);
             atomic_add_u64(&a.misc[kMiscReplayMismatch], wc[wMismatch]);
+            atomic_add_u64(&a.misc[kMiscMarches], wc[wMarch]);
         }
     }
+#undef a
 }
 
 // ---------------------------------------------------------------------------

@@ -124,3 +124,22 @@ def test_batch_rejects_oversized_tables():
     with pytest.raises(lp.LPError):
         ctx.run_batch(tab, 0, 10)
     ctx.close()
+
+
+def test_batch_against_oracle():
+    # each instance of a batch against the oracle with that instance's
+    # optics, same seed and photon range: detector and region tallies bit
+    # for bit
+    import _oracle as O
+    ctx, sc = _ctx(seed=41)
+    tab = _table([(660.0, 0.97, 0.5), (940.0, 0.9, 0.3), (660.0, 0.8, 0.7)])
+    first, count = 4321, 300
+    b = ctx.run_batch(tab, first, count)
+    ctx.close()
+    c = 2.0 + 8.0 * np.arange(8) / 7.0
+    for i, nm in enumerate(tab):
+        run = O.Run(dict(sc, nmuasg=nm), seed=41, tallies={"det_rlo": c - 0.25, "det_rhi": c + 0.25})
+        run.run(first, count, nthreads=8)
+        t = run.tallies()
+        assert np.array_equal(b["det"][i], t["det"].reshape(-1, 2)), i
+        assert np.array_equal(b["region"][i], t["region"]), i

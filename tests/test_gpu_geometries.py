@@ -109,3 +109,62 @@ def test_geometry_errors():
     with pytest.raises(lp.LPError):
         ctx.run_batch(_table()[:2], 0, 10, geom=[0, 1])
     ctx.close()
+
+
+def test_batched_geometries_against_oracle():
+    # the batched launch over three merged geometries against the oracle run
+    # on each instance's own geometry with that instance's optics: detector
+    # and region tallies bit for bit, for 200 photons per instance
+    import _oracle as O
+    scs = _scenes()
+    merged, counts = S.geometries(scs)
+    tab = _table()
+    geom = np.array([0, 1, 2, 2, 0, 1, 1], np.int32)
+    first, count = 777, 200
+    ctx = _ctx(merged, 0x5EED0011, counts)
+    g = ctx.run_batch(tab, first, count, geom=geom)
+    ctx.close()
+    c = 2.0 + 8.0 * np.arange(8) / 7.0
+    for i in range(len(tab)):
+        sc = dict(scs[geom[i]], nmuasg=tab[i])
+        run = O.Run(sc, seed=0x5EED0011, tallies={"det_rlo": c - 0.25, "det_rhi": c + 0.25})
+        run.run(first, count, nthreads=8)
+        t = run.tallies()
+        print("instance", i, "geometry", geom[i], "R", t["det"].reshape(-1, 2)[:, 0].sum(), flush=True)
+        assert np.array_equal(g["det"][i], t["det"].reshape(-1, 2)), i
+        assert np.array_equal(g["region"][i], t["region"]), i
+
+
+C2_GEOMS = [(1.5, 19, 14, {"t_fat": 0.3, "t_uterus": 0.5}), (2.5, 38, 27, {}), (3.5, 45, 32, {}), (4.5, 53, 38, {})]
+
+
+def test_batched_workload_at_full_size():
+    # the batched workload's real shape -- 4 merged geometries (56 480 to
+    # 377 120 faces), 2 wavelengths x 16 fetal SpO2 per geometry = 128
+    # instances in one launch -- with 20 000 photons per instance: every
+    # instance equals lp_run of its optics on its geometry alone
+    spo2 = np.linspace(0.30, 0.75, 16)
+    scs = [S.maternal_fetal(fetal_depth=d, freq_maternal=fm, freq_fetal=ff, **kw) for d, fm, ff, kw in C2_GEOMS]
+    merged, counts = S.geometries(scs)
+    assert counts == [56_480, 196_420, 272_400, 377_120]
+    one = np.stack([S.mf_optics(wl, 0.97, sf) for wl in (660.0, 940.0) for sf in spo2])
+    tab = np.concatenate([one] * 4)
+    geom = np.repeat(np.arange(4, dtype=np.int32), len(one))
+    first, count = 0, 20_000
+    ctx = _ctx(merged, 0x5EED0002, counts)
+    print("merged", counts, "instances", len(tab), flush=True)
+    g = ctx.run_batch(tab, first, count, geom=geom)
+    assert ctx.stats().photons == 128 * count
+    ctx.close()
+    for k, sc in enumerate(scs):
+        one_ctx = _ctx(sc, 0x5EED0002)
+        for i in np.nonzero(geom == k)[0]:
+            one_ctx.set_materials(tab[i], sc["fetal_id"])
+            one_ctx.reset()
+            one_ctx.run(first, count)
+            t = one_ctx.tallies()
+            assert np.array_equal(g["det"][i], t["det"].reshape(-1, 2)), (k, i)
+            assert np.array_equal(g["region"][i], t["region"]), (k, i)
+        print("geometry", k, "done", flush=True)
+        one_ctx.close()
+    assert g["det"][:, :, 0].sum() > 0

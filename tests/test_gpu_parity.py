@@ -269,3 +269,50 @@ def test_face_met_at_exactly_the_step_length():
             ex = ctx.exit_records(1)[0]
             assert ex.view(np.uint32).tolist() == bits, (opts, pid, ex.tolist())
         ctx.close()
+
+
+@pytest.mark.parametrize("opts", [{}, {"near_r0": 0.0}, {"no_skip": 1}], ids=["default", "bvh_only", "no_skip"])
+def test_headline_mesh_against_oracle_golden(opts):
+    # 2048 photons of the headline workload (the ten-region maternal-fetal
+    # mesh, 196 420 faces, 660 nm, bench.setup tallies) against the oracle's
+    # results committed by tests/golden/make_c1_golden.py: every exit
+    # record bit for bit, every nonzero of the SFT, RTd, region, radial and
+    # detector tallies, the voxel tally's hash, and the physics counters --
+    # with the default search shortcuts, with the BVH alone, and with every
+    # step searched
+    import hashlib
+    import json
+    import os
+    from lightpropagation_amd import scenes as S
+    import bench
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c1_photons.json")))
+    sc = S.maternal_fetal()
+    assert len(sc["tri"]) == g["faces"]
+    ctx = lp.Context(0)
+    for k, v in opts.items():
+        ctx.set_option(k, v)
+    bench.setup(ctx, sc, seed=g["seed"])
+    ctx.set_tallies(sum(1 << lp.TALLY[k] for k in ("sft", "rtd", "region", "vox", "rad", "det", "misc")),
+                    record_exit=True)
+    ctx.reset()
+    print("options", opts, flush=True)
+    ctx.run(g["first"], g["count"])
+    ex = ctx.exit_records(g["count"])
+    t = ctx.tallies()
+    ctx.close()
+    want = np.array(g["exit_bits"], np.uint32)
+    bad = np.nonzero(np.any(ex.view(np.uint32) != want, axis=1))[0]
+    assert len(bad) == 0, (len(bad), bad[:5])
+    for k in ("sft", "rtd", "region", "rad", "det"):
+        a = np.asarray(t[k]).reshape(-1)
+        assert len(a) == g[k]["n"], k
+        nz = np.nonzero(a)[0]
+        assert nz.tolist() == g[k]["index"] and a[nz].tolist() == g[k]["value"], k
+    v = np.ascontiguousarray(t["vox"], np.int64)
+    assert int(np.count_nonzero(v)) == g["vox"]["nonzero"] and int(v.sum()) == g["vox"]["sum"]
+    assert hashlib.sha256(v.tobytes()).hexdigest() == g["vox"]["sha256"]
+    for k, val in g["misc"].items():
+        if k != "arz_out":  # the Arz image is not among the tallies kept here
+            assert t["misc"][k] == val, k
+    if opts.get("no_skip"):
+        assert t["misc"]["skipped"] == 0

@@ -115,3 +115,14 @@ def test_without_near_grid():
     assert np.array_equal(a["nodes"], b["nodes"]) and np.array_equal(a["ids"], b["ids"])
     assert np.array_equal(a["roots"], b["roots"])
     assert b["sizes"][2] == 0 and b["sizes"][3] == 0 and np.all(b["coffs"] == 0)
+
+
+def test_shared_frame_fits_every_geometry():
+    # the shared near-grid frame is fitted to the vertices each geometry's
+    # grid sees (triangle records in float); fitted to the input vertices, it
+    # missed one geometry's box by a rounding at this radius
+    GEOMS = [(1.5, 8, {"t_fat": 0.3, "t_uterus": 0.5}), (2.5, 10, {}), (3.5, 6, {})]
+    scs = [S.maternal_fetal(fetal_depth=d, freq_maternal=f, freq_fetal=f, freq_torso=4, **kw) for d, f, kw in GEOMS]
+    for r0 in (0.05, 0.07):
+        r = _build(scs, 85, r0, 2 * r0)
+        assert r["sizes"][4] > 0

@@ -590,7 +590,10 @@ int lp_set_voxel_grid(lp_context* c, const float origin[3], const float size[3],
     if (!c) return LP_EINVAL;
     if (!origin || !size || !n) return fail(c, LP_EINVAL, "lp_set_voxel_grid: null");
     for (int k = 0; k < 3; ++k)
-        if (n[k] < 0 || (n[k] > 0 && !(size[k] > 0))) return fail(c, LP_EINVAL, "lp_set_voxel_grid: bad size");
+        if (n[k] < 0 || n[k] > (1 << 24) || (n[k] > 0 && !(size[k] > 0)))
+            return fail(c, LP_EINVAL, "lp_set_voxel_grid: bad size (at most 2^24 voxels per axis)");
+    if ((int64_t)n[0] * n[1] * n[2] >= ((int64_t)1 << 31))
+        return fail(c, LP_EINVAL, "lp_set_voxel_grid: fewer than 2^31 voxels");
     std::memcpy(c->vox_origin, origin, sizeof c->vox_origin);
     std::memcpy(c->vox_size, size, sizeof c->vox_size);
     std::memcpy(c->vox_n, n, sizeof c->vox_n);

@@ -363,7 +363,8 @@ __device__ __forceinline__ float mesh_distance(const KArgs& a, const float4* lno
 // Cell of the near-face grid holding p, or -1 outside it (then no face lies
 // within near_r0 of p), in the frame all geometries share (callers add the
 // cell offset of the photon's geometry).
-__device__ __forceinline__ int64_t near_cell(const KArgs& a, f3 p)
+// (The grid holds fewer than 2^31 cells, so 32-bit index arithmetic.)
+__device__ __forceinline__ int32_t near_cell(const KArgs& a, f3 p)
 {
     float fx = (p.x - a.ng_origin[0]) * a.ng_inv[0];
     float fy = (p.y - a.ng_origin[1]) * a.ng_inv[1];
@@ -371,7 +372,7 @@ __device__ __forceinline__ int64_t near_cell(const KArgs& a, f3 p)
     if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f)) return -1;
     int32_t ix = (int32_t)fminf(fx, 2.0e9f), iy = (int32_t)fminf(fy, 2.0e9f), iz = (int32_t)fminf(fz, 2.0e9f);
     if (ix >= a.ng_n[0] || iy >= a.ng_n[1] || iz >= a.ng_n[2]) return -1;
-    return ((int64_t)iz * a.ng_n[1] + iy) * a.ng_n[0] + ix;
+    return (iz * a.ng_n[1] + iy) * a.ng_n[0] + ix;
 }
 
 // BVH root and near-grid cell offset of an instance's geometry: the int bits
@@ -419,9 +420,9 @@ __device__ __forceinline__ void near_batch(const KArgs& a, bool part, uint32_t c
     unlisted = a.near_r0;  // the grid reaches r0 beyond every face
     uint32_t b = 0, n = 0;
     if (part) {
-        int64_t c = near_cell(a, cq);
-        if (c >= 0) {
-            c += coff;
+        const int32_t c0 = near_cell(a, cq);
+        if (c0 >= 0) {
+            const uint32_t c = (uint32_t)c0 + coff;
             const CellRec cr = a.near_cells[c];
             b = cr.first;
             n = a.near_cells[c + 1].first - b;
@@ -651,7 +652,7 @@ __device__ __forceinline__ void jac_dda(const KArgs& a, unsigned long long* jd, 
 // per-wavefront event counters in LDS
 enum WaveCount {
     wPhot, wInt, wBnd, wEsc, wCap, wSkip, wDq, wNan, wLst, wIter, wLRounds, wBRounds, wArzOut, wVoxOut, wMismatch,
-    wMarch, kWc
+    wMarch, wVis, wTst, wDvis, kWc
 };
 
 // WAVES: minimum waves per SIMD the register allocator must allow (1 =
@@ -740,9 +741,15 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
         const unsigned long long m = __ballot(1);
         if (lane == __ffsll((long long)m) - 1) atomicAdd(&wc[slot], (unsigned long long)__popcll(m));
     };
-    uint32_t lvis = 0, ltst = 0, ldvis = 0;  // BVH node visits, triangle tests, distance-query visits
+    // the frequent events are counted per round in wave-uniform registers
+    // (ballot popcounts) and added to the LDS row once per round
+    uint32_t r_int = 0, r_bnd = 0, r_skip = 0, r_iter = 0, r_lr = 0, r_lst = 0;
+    // adds a lane's own count (search statistics of the rare BVH batches)
+    auto wsum = [&](int slot, uint32_t v) {
+        if (v) atomicAdd(&wc[slot], (unsigned long long)v);
+    };
     unsigned long long reg_acc = 0;  // fixed-point absorption in ph.curr not yet flushed
-    int64_t vox_cell = -1;           // voxel whose deposits are pending in vox_acc
+    int32_t vox_cell = -1;           // voxel whose deposits are pending in vox_acc
     unsigned long long vox_acc = 0;
 
     float stepsize = 0.0f;  // drawn step length, pending until used
@@ -856,19 +863,24 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             float fx = (c.x - a.vox_origin[0]) * a.vox_inv[0];
             float fy = (c.y - a.vox_origin[1]) * a.vox_inv[1];
             float fz = (c.z - a.vox_origin[2]) * a.vox_inv[2];
-            bool ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f;
-            int32_t ix = cvt_i32f(fx), iy = cvt_i32f(fy), iz = cvt_i32f(fz);
-            ok = ok && ix >= 0 && iy >= 0 && iz >= 0 && ix < a.vox_n[0] && iy < a.vox_n[1] && iz < a.vox_n[2];
+            // the truncation lies in [0, n) exactly when x does (n < 2^24 is
+            // exact in float): the host's int() binning with its bounds check
+            const bool ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)a.vox_n[0] &&
+                            fy < (float)a.vox_n[1] && fz < (float)a.vox_n[2];
             // consecutive deposits in one voxel are summed in a register
-            // (value selects, so the accumulator stays in registers)
-            int64_t cell = ((int64_t)iz * a.vox_n[1] + iy) * a.vox_n[0] + ix;
+            // (value selects, so the accumulator stays in registers); fewer
+            // than 2^31 voxels, so 32-bit indices
+            // (clamped first, so that the conversions are defined off the grid too)
+            const int32_t ix = (int32_t)__builtin_amdgcn_fmed3f(fx, 0.0f, 16777216.0f);
+            const int32_t iy = (int32_t)__builtin_amdgcn_fmed3f(fy, 0.0f, 16777216.0f);
+            const int32_t iz = (int32_t)__builtin_amdgcn_fmed3f(fz, 0.0f, 16777216.0f);
+            const int32_t cell = (iz * a.vox_n[1] + iy) * a.vox_n[0] + ix;
             bool same = ok && cell == vox_cell;
             if (ok && !same && vox_cell >= 0) atomic_add_u64(&a.vox[vox_cell], vox_acc);
             vox_acc = same ? vox_acc + q : (ok ? q : vox_acc);
             vox_cell = ok ? cell : vox_cell;
             if (!ok && q) atomicAdd(&wc[wVoxOut], q);
         }
-        wcount(wInt);
     };
 
     // boundary branch (.cl:1462-1511) at face h.id, distance stb
@@ -966,7 +978,6 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             if (ph.curr == a.fetal_id) ph.fetal |= 1;
             ph.unfin = (s - stb) * (cm.y + cm.z);
         }
-        wcount(wBnd);
         return kind;
     };
 
@@ -1116,12 +1127,6 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                 }
             }
         }
-        if ((lvis | ltst | ldvis) > 0x40000000u && a.misc) {  // rare: keep the 32-bit counts from wrapping
-            atomic_add_u64(&a.misc[kMiscNodeVisits], lvis);
-            atomic_add_u64(&a.misc[kMiscTriTests], ltst);
-            atomic_add_u64(&a.misc[kMiscDistVisits], ldvis);
-            lvis = ltst = ldvis = 0;
-        }
     };
 
     for (;;) {
@@ -1206,6 +1211,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             launder();
             bool go = false;     // this lane takes a step in this iteration
             bool freed = false;  // ... and the step was shown to stay in the free region
+            bool skipped = false;  // ... with no face search
             bool bnd = false;
             Hit h;
             h.id = -1;
@@ -1240,9 +1246,10 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                     // else the ball of the near-grid cell's clearance around p:
                     // no face lies within the clearance of any point the cell
                     // lookup places in the cell
-                    const int64_t c = near_cell(a, ph.p);
+                    const int32_t c = near_cell(a, ph.p);
                     if (c >= 0) {
-                        const float cl = __uint_as_float(a.near_cells[c + geom_of<MG>(a, lmat, ph.inst).coff].clear_bits);
+                        const float cl =
+                            __uint_as_float(a.near_cells[(uint32_t)c + geom_of<MG>(a, lmat, ph.inst).coff].clear_bits);
                         const float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
                                                 __builtin_fabsf(ph.p.z));
                         const float sc = cl - (a.safe_margin + 1e-5f * (cl + rad));
@@ -1258,7 +1265,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                 }
                 if (freed || (!a.no_skip && !ballistic && a.safety && stepsize < safety_at(a, ph.p))) {
                     go = true;
-                    wcount(wSkip);
+                    skipped = true;
                 } else {
                     search = true;
                     mq = 0.0f;
@@ -1277,9 +1284,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             const bool at_cap = it + 1 >= a.free_iters;
             const bool do_list = nlist > 0 && (stuck || at_cap || nlist >= a.list_min * nlive);
             const bool do_bvh = nbvh > 0 && (stuck || at_cap || nbvh >= a.search_min * nlive);
-            if (lane == 0) atomicAdd(&wc[wIter], 1ull);
+            ++r_iter;
             if (do_list) {
-                if (lane == 0) atomicAdd(&wc[wLRounds], 1ull);
+                ++r_lr;
                 // exact answer from the cells' lists, and the free region with it
                 const bool part = search && by_list;
                 float d1sq, d2sq, unlisted;
@@ -1303,7 +1310,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                 Hit hl;
                 uint32_t pairs;
                 near_batch(a, part, coff, cq, ph.p, ph.v, ph.ref, wmax, want, hl, d1sq, d2sq, i1, unlisted, pairs);
-                if (lane == 0) atomicAdd(&wc[wLst], (unsigned long long)pairs);
+                r_lst += pairs;
                 if (part && reach) {
                     if (hl.id >= 0) {  // the nearest face ahead
                         h = hl;
@@ -1326,12 +1333,12 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                         while (cleared < tlim && n < 256) {
                             ++n;
                             const f3 q = add(ph.p, mul(cleared, ph.v));
-                            const int64_t c = near_cell(a, q);
+                            const int32_t c = near_cell(a, q);
                             if (c < 0) {
                                 exited = true;
                                 break;
                             }
-                            const float cl = __uint_as_float(a.near_cells[c + coff].clear_bits);
+                            const float cl = __uint_as_float(a.near_cells[(uint32_t)c + coff].clear_bits);
                             const float rq =
                                 fmaxf(fmaxf(__builtin_fabsf(q.x), __builtin_fabsf(q.y)), __builtin_fabsf(q.z));
                             const float sc = cl - (a.safe_margin + 1e-5f * (cl + rq));
@@ -1406,7 +1413,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                 if (__any(dq)) {
                     if (dq) {
                         float cap = fminf(a.dist_cap, a.dist_cap_steps * stepsize);
-                        float d = mesh_distance(a, lnodes, geom_of<MG>(a, lmat, ph.inst).root, ph.p, cap, ldvis);
+                        uint32_t dvis = 0;
+                        float d = mesh_distance(a, lnodes, geom_of<MG>(a, lmat, ph.inst).root, ph.p, cap, dvis);
+                        wsum(wDvis, dvis);
                         float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
                                           __builtin_fabsf(ph.p.z));
                         ph.safe = fmaxf(0.0f, d - (a.safe_margin + 1e-5f * (d + rad)));  // >= 0: the test squares it
@@ -1417,14 +1426,17 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                         if (stepsize < ph.safe) {
                             search = false;
                             go = true;
-                            wcount(wSkip);
+                            skipped = true;
                         }
                     }
                 }
                 // nearest face within the pending step
                 if (search && !by_list) {
+                    uint32_t vis = 0, tst = 0;
                     find_face(a, lnodes, geom_of<MG>(a, lmat, ph.inst).root, ph.p, ph.v, ph.ref,
-                              ballistic ? 10000.0f : stepsize, h, lvis, ltst);
+                              ballistic ? 10000.0f : stepsize, h, vis, tst);
+                    wsum(wVis, vis);
+                    wsum(wTst, tst);
                     bnd = h.id >= 0 || ballistic || stepsize >= 10000.0f;  // no face ahead at all: escape
                     if (h.id < 0) h.t = 10000.0f;
                     if (ballistic) stepsize = h.t;
@@ -1451,20 +1463,24 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                     alive = false;
                 }
             }
+            r_int += (uint32_t)__popcll(__ballot(go && !bnd));
+            r_bnd += (uint32_t)__popcll(__ballot(go && bnd));
+            r_skip += (uint32_t)__popcll(__ballot(skipped));
             // end the round at the cap (finished lanes are refilled then), or
             // once enough lanes have finished and photons are left to start
             if (at_cap || (!exhausted && 64 - __popcll(__ballot(alive)) >= a.refill_min)) break;
         }
+        if (lane == 0) {  // the round's counts into the wavefront's row
+            wc[wInt] += r_int;
+            wc[wBnd] += r_bnd;
+            wc[wSkip] += r_skip;
+            wc[wIter] += r_iter;
+            wc[wLRounds] += r_lr;
+            wc[wLst] += r_lst;
+        }
+        r_int = r_bnd = r_skip = r_iter = r_lr = r_lst = 0;
     }
     if (a.misc) {
-        const uint32_t lane_cnt[3] = {lvis, ltst, ldvis};
-        const int lane_slot[3] = {kMiscNodeVisits, kMiscTriTests, kMiscDistVisits};
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            uint64_t v = lane_cnt[i];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) atomic_add_u64(&a.misc[lane_slot[i]], v);
-        }
         __threadfence_block();
         if (lane == 0) {
             atomic_add_u64(&a.misc[kMiscPhotons], wc[wPhot]);
@@ -1484,6 +1500,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             atomic_add_u64(&a.misc[kMiscVoxOut], wc[wVoxOut]);
             atomic_add_u64(&a.misc[kMiscReplayMismatch], wc[wMismatch]);
             atomic_add_u64(&a.misc[kMiscMarches], wc[wMarch]);
+            atomic_add_u64(&a.misc[kMiscNodeVisits], wc[wVis]);
+            atomic_add_u64(&a.misc[kMiscTriTests], wc[wTst]);
+            atomic_add_u64(&a.misc[kMiscDistVisits], wc[wDvis]);
         }
     }
 #undef a

@@ -208,6 +208,20 @@ int lp_adjoint(lp_context *ctx, int64_t first, int64_t count, int64_t *signals, 
  * jac or jac_mus may be NULL. */
 int lp_adjoint_voxels(lp_context *ctx, int64_t first, int64_t count, int64_t *signals, int64_t *grad, int64_t *jac,
                       int64_t *jac_mus, int64_t *npairs);
+/* Per-pair path records of the adjoint (no reference counterpart): with
+ * enable != 0, every later lp_adjoint / lp_adjoint_voxels keeps, for each
+ * (photon, detector) pair it replays, the photon's path length and
+ * interaction count in each tissue (4 + 4 bytes per tissue and pair on the
+ * device until the next call); enable = 0 frees them.  These are the
+ * perturbation Monte Carlo records: the pair's weight at other optics is
+ *   W' = W prod_t (mus'_t / mus_t)^n_t exp(-(mua'_t + mus'_t - mua_t - mus_t) L_t). */
+int lp_set_adjoint_paths(lp_context *ctx, int32_t enable);
+/* Copies the records of the last lp_adjoint out, sorted by (photon,
+ * detector): photon[npairs], det[npairs], w[npairs] (R exit weight),
+ * length[npairs][ntissue], interactions[npairs][ntissue].  npairs must be
+ * the count that call returned. */
+int lp_get_adjoint_paths(lp_context *ctx, int64_t npairs, int64_t *photon, int32_t *det, float *w, float *length,
+                         uint32_t *interactions);
 
 /* Evaluates the kernel's math core element-wise on the device (op 0 sqrt,
  * 1 divide, 2 log, 3 cos, 4 acos, 5 multiply, 6 add) over n (x, y) pairs.

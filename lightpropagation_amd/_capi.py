@@ -77,6 +77,8 @@ _SIGS = {
     "lp_set_jacobian_grid": (C.c_int, [C.c_void_p, _fp, _fp, _ip]),
     "lp_adjoint": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _lp, _lp, _lp, _lp]),
     "lp_adjoint_voxels": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _lp, _lp, _lp, _lp, _lp]),
+    "lp_set_adjoint_paths": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lp_get_adjoint_paths": (C.c_int, [C.c_void_p, C.c_int64, _lp, _ip, _fp, _fp, C.c_void_p]),
     "lp_run_batch": (C.c_int, [C.c_void_p, _fp, C.c_int32, C.c_int64, C.c_int64, _lp, _lp]),
     "lp_set_geometries": (C.c_int, [C.c_void_p, _lp, C.c_int32]),
     "lp_run_batch_geom": (C.c_int, [C.c_void_p, _fp, C.c_int32, _ip, C.c_int64, C.c_int64, _lp, _lp]),
@@ -255,11 +257,14 @@ class Context:
         self._chk(lib().lp_set_jacobian_grid(self._c, fp(_f(origin)), fp(_f(size)), ip(_i(n))),
                   "lp_set_jacobian_grid")
 
-    def adjoint(self, first, count, jac=False, ntissue=None, ndet=None, jac_mus=False):
+    def adjoint(self, first, count, jac=False, ntissue=None, ndet=None, jac_mus=False, paths=False):
         """lp_adjoint (lp_adjoint_voxels with jac_mus): dict(signals (ndet,),
         grad (ndet, ntissue, 2), jac (ndet, nz, ny, nx) or None, jac_mus
         (same shape) or None, pairs) as int64 fixed point (2^32 per unit);
-        the photons also add to the tallies."""
+        the photons also add to the tallies.  paths: also the per-pair path
+        records (lp_set_adjoint_paths / lp_get_adjoint_paths) as dict(photon,
+        det, w, length (npairs, ntissue), interactions (npairs, ntissue))
+        under "paths"."""
         nt = ntissue if ntissue is not None else self._keep["ntissue"]
         nd = ndet if ndet is not None else self._keep["ndet"]
         sig = np.zeros(nd, np.int64)
@@ -273,11 +278,22 @@ class Context:
             nx, ny, nz = getattr(self, "_jac_n", (0, 0, 0))
             Js = np.zeros((nd, nz, ny, nx), np.int64)
         pairs = C.c_int64()
+        self._chk(lib().lp_set_adjoint_paths(self._c, int(bool(paths))), "lp_set_adjoint_paths")
         self._chk(lib().lp_adjoint_voxels(self._c, int(first), int(count), sig.ctypes.data_as(_lp),
                                           grad.ctypes.data_as(_lp), J.ctypes.data_as(_lp) if J is not None else None,
                                           Js.ctypes.data_as(_lp) if Js is not None else None, C.byref(pairs)),
                   "lp_adjoint_voxels")
-        return dict(signals=sig, grad=grad, jac=J, jac_mus=Js, pairs=pairs.value)
+        out = dict(signals=sig, grad=grad, jac=J, jac_mus=Js, pairs=pairs.value)
+        if paths:
+            n = pairs.value
+            P = dict(photon=np.zeros(n, np.int64), det=np.zeros(n, np.int32), w=np.zeros(n, np.float32),
+                     length=np.zeros((n, nt), np.float32), interactions=np.zeros((n, nt), np.uint32))
+            self._chk(lib().lp_get_adjoint_paths(self._c, n, P["photon"].ctypes.data_as(_lp), ip(P["det"]),
+                                                 fp(P["w"]), fp(P["length"]), P["interactions"].ctypes.data),
+                      "lp_get_adjoint_paths")
+            self._chk(lib().lp_set_adjoint_paths(self._c, 0), "lp_set_adjoint_paths")
+            out["paths"] = P
+        return out
 
     def run_batch(self, nmuasg, first, count, geom=None):
         """lp_run_batch (geom None) or lp_run_batch_geom (geom: geometry of

@@ -91,6 +91,11 @@ struct lp_context {
     DevBuf d_tune[LP_NTALLY];  // the autotuner's stand-ins for the tallies (same sizes)
     DevBuf d_det_list, d_det_count, d_grad, d_jac, d_jac_coll, d_adj_misc;
     int64_t exit_count = 0;
+    // per-pair path records of the last lp_adjoint (lp_set_adjoint_paths)
+    int32_t adj_paths = 0;
+    DevBuf d_pair_len, d_pair_int;
+    std::vector<DetRec> pair_list;  // the replayed pairs, in the order of the records
+    int32_t pair_nt = 0;
     lp_stats stats{};
 
     // batched launches: instance material table, per-instance tallies, and
@@ -482,7 +487,7 @@ void lp_destroy(lp_context* c)
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
                       &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list,
                       &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_jac_coll, &c->d_adj_misc, &c->d_inst_mat,
-                      &c->d_bdet, &c->d_bregion};
+                      &c->d_bdet, &c->d_bregion, &c->d_pair_len, &c->d_pair_int};
     for (DevBuf* b : bufs) dev_free(*b);
     for (auto& b : c->d_tally) dev_free(b);
     for (auto& b : c->d_tune) dev_free(b);
@@ -1232,6 +1237,8 @@ int lp_adjoint_voxels(lp_context* c, int64_t first, int64_t count, int64_t* sign
         overlap = std::max(overlap, o);
     }
     const int64_t cap = std::max<int64_t>(1, count * std::max(overlap, 1));
+    c->pair_list.clear();
+    c->pair_nt = 0;
 
     // 1. forward, recording the detected photons
     if ((rc = dev_alloc(c, c->d_det_list, (size_t)cap * sizeof(DetRec)))) return rc;
@@ -1295,6 +1302,12 @@ int lp_adjoint_voxels(lp_context* c, int64_t first, int64_t count, int64_t* sign
         b.jac_inv[k] = 1.0f / c->jac_size[k];
         b.jac_n[k] = c->jac_n[k];
     }
+    if (c->adj_paths && n > 0) {
+        const size_t pbytes = (size_t)n * nt * sizeof(float);
+        if ((rc = dev_alloc(c, c->d_pair_len, pbytes)) || (rc = dev_alloc(c, c->d_pair_int, pbytes))) return rc;
+        b.pair_len = (float*)c->d_pair_len.p;
+        b.pair_int = (uint32_t*)c->d_pair_int.p;
+    }
     double rep_ms = 0;
     if (n > 0) {
         if ((rc = launch(c, b))) return rc;
@@ -1317,11 +1330,66 @@ int lp_adjoint_voxels(lp_context* c, int64_t first, int64_t count, int64_t* sign
     if (want_mus && jac)
         for (int64_t i = 0; i < (int64_t)nd * nvox; ++i) jac_mus[i] += jac[i];
     if (npairs) *npairs = (int64_t)n;
+    if (c->adj_paths) {
+        c->pair_list = std::move(list);
+        c->pair_nt = nt;
+    }
     c->stats.kernel_ms = fwd_ms;
     c->stats.photons = count;
     c->stats.replay_ms = rep_ms;
     c->stats.replayed = (int64_t)n;
     c->stats.replay_mismatch = misc[kMiscReplayMismatch];
+    return LP_OK;
+}
+
+int lp_set_adjoint_paths(lp_context* c, int32_t enable)
+{
+    if (!c) return LP_EINVAL;
+    c->adj_paths = enable != 0;
+    if (!c->adj_paths) {
+        dev_free(c->d_pair_len);
+        dev_free(c->d_pair_int);
+        c->pair_list.clear();
+        c->pair_list.shrink_to_fit();
+        c->pair_nt = 0;
+    }
+    return LP_OK;
+}
+
+int lp_get_adjoint_paths(lp_context* c, int64_t npairs, int64_t* photon, int32_t* det, float* w, float* length,
+                         uint32_t* interactions)
+{
+    if (!c) return LP_EINVAL;
+    if (!c->adj_paths) return fail(c, LP_ESTATE, "lp_get_adjoint_paths: path records are off (lp_set_adjoint_paths)");
+    const int64_t n = (int64_t)c->pair_list.size();
+    if (npairs != n)
+        return fail(c, LP_EINVAL, "lp_get_adjoint_paths: the last lp_adjoint replayed %lld pairs, not %lld",
+                    (long long)n, (long long)npairs);
+    if (n == 0) return LP_OK;
+    if (!photon || !det || !w || !length || !interactions) return fail(c, LP_EINVAL, "lp_get_adjoint_paths: null");
+    const int32_t nt = c->pair_nt;
+    std::vector<float> L((size_t)n * nt);
+    std::vector<uint32_t> N((size_t)n * nt);
+    HIPCHK(c, hipMemcpyAsync(L.data(), c->d_pair_len.p, L.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(N.data(), c->d_pair_int.p, N.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the detection list holds the pairs in the order the forward launch
+    // found them, which varies from run to run: hand them out by (photon, detector)
+    std::vector<int64_t> order((size_t)n);
+    for (int64_t i = 0; i < n; ++i) order[i] = i;
+    const std::vector<DetRec>& r = c->pair_list;
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+        return r[x].photon != r[y].photon ? r[x].photon < r[y].photon : r[x].det < r[y].det;
+    });
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t i = order[j];
+        photon[j] = r[i].photon;
+        det[j] = r[i].det;
+        w[j] = r[i].w;
+        std::memcpy(length + j * nt, L.data() + i * nt, (size_t)nt * sizeof(float));
+        std::memcpy(interactions + j * nt, N.data() + i * nt, (size_t)nt * sizeof(uint32_t));
+    }
     return LP_OK;
 }
 
@@ -1332,7 +1400,7 @@ int lp_get_stats(lp_context* c, lp_stats* out)
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
                       &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list,
                       &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_jac_coll, &c->d_adj_misc, &c->d_inst_mat,
-                      &c->d_bdet, &c->d_bregion};
+                      &c->d_bdet, &c->d_bregion, &c->d_pair_len, &c->d_pair_int};
     for (DevBuf* b : bufs) bytes += (int64_t)b->bytes;
     for (auto& b : c->d_tally) bytes += (int64_t)b.bytes;
     c->stats.device_bytes = bytes;

@@ -196,6 +196,11 @@ struct KArgs {
                                    // (null = none); d/dmus of voxel v = jac[v] + jac_coll[v]
     float jac_origin[3], jac_inv[3];  // jac_inv = 1 / voxel size in float
     int32_t jac_n[3];
+    // per-pair path records of the replay (null = none): the path length and
+    // interaction count in each tissue of the photon of adj_list[k], at
+    // [k * ntissue + t]
+    float* pair_len;
+    uint32_t* pair_int;
     // batched launch (inst_mat null = one instance with nmuasg and rsp):
     // ninst instances differing in their tissue optics, instance i's
     // materials at inst_mat[i * (ntissue + 1) + t] and its (rsp, root, coff,

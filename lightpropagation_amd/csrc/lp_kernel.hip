@@ -764,6 +764,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
     f3 xsft = mk(0.0f, 0.0f, 0.0f);
     int32_t adj_det = 0;  // adjoint: the detector and R weight of the replayed photon
     float adj_w = 0.0f;
+    int64_t adj_k = 0;    // ... and its index in the detection list
     // adjoint: one move of length s from p0 in the current tissue, ending in
     // an interaction (collision) or at a face
     auto adj_step = [&](f3 p0, float s, bool collision, float mus) {
@@ -1035,6 +1036,10 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             for (int32_t t = 0; t < a.ntissue; ++t) {
                 const float L = adjL[t * kBlock + tid];
                 const uint32_t n = adjN[t * kBlock + tid];
+                if (a.pair_len) {
+                    a.pair_len[adj_k * a.ntissue + t] = L;
+                    a.pair_int[adj_k * a.ntissue + t] = n;
+                }
                 if (L == 0.0f && n == 0u) continue;
                 const float mus = lmat[t].z;
                 const float gm = -(L * W);
@@ -1156,6 +1161,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                             id = r.photon;
                             adj_det = r.det;
                             adj_w = r.w;
+                            adj_k = (int64_t)k;
                             for (int32_t t = 0; t < a.ntissue; ++t) {
                                 adjL[t * kBlock + tid] = 0.0f;
                                 adjN[t * kBlock + tid] = 0u;

@@ -81,15 +81,16 @@ class Server:
         self.ctx.set_materials(np.asarray(nmuasg, np.float32), self.sc["fetal_id"])
         return None
 
-    def adjoint(self, first, count, jac=False, nmuasg=None):
+    def adjoint(self, first, count, jac=False, nmuasg=None, paths=False):
         """Per-photon detector signals and gradients (float64, divided by the
-        photon count) of photons [first, first + count)."""
+        photon count) of photons [first, first + count); paths: also the
+        per-pair path records (lp_get_adjoint_paths) as they come."""
         import lightpropagation_amd as lp
         if nmuasg is not None:
             self.materials(nmuasg)
         self.ctx.reset()
         t0 = time.perf_counter()
-        r = self.ctx.adjoint(first, count, jac=jac)
+        r = self.ctx.adjoint(first, count, jac=jac, paths=paths)
         wall = time.perf_counter() - t0
         st = self.ctx.stats()
         if st.replay_mismatch:
@@ -97,7 +98,7 @@ class Server:
         s = lp.FIXED * float(count)
         return dict(signals=r["signals"] / s, grad=r["grad"] / s,
                     jac=None if r["jac"] is None else r["jac"] / s, pairs=r["pairs"],
-                    forward_ms=st.kernel_ms, replay_ms=st.replay_ms, wall_s=wall)
+                    forward_ms=st.kernel_ms, replay_ms=st.replay_ms, wall_s=wall, paths=r.get("paths"))
 
     def run(self, first, count, nmuasg=None):
         """Per-photon detector signals of a forward run (float64)."""

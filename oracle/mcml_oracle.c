@@ -965,6 +965,12 @@ int orc_adjoint(const orc_params *P, int64_t first, int64_t count, orc_tallies *
             A.nseg = 0;
             run_photon(P, &L, first + i, TT, reg_l, misc_l, NULL, NULL, 0, NULL, &A);
             const float W = A.L[nt], radius = A.L[nt + 1];
+            if (out->path_len) {
+                memcpy(out->path_len + i * nt, A.L, (size_t)nt * sizeof(float));
+                memcpy(out->path_int + i * nt, A.n, (size_t)nt * sizeof(int32_t));
+                out->path_w[i] = W;
+                out->path_r[i] = radius;
+            }
             if (W == 0.0f) continue;
             for (int32_t d = 0; d < P->ndet; ++d) {
                 if (!(radius >= P->det_rlo[d] && radius < P->det_rhi[d])) continue;

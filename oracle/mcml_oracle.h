@@ -190,6 +190,14 @@ typedef struct orc_adjoint_out {
      * (sum over the interactions in v of 1 / mus) W - L_v W, accumulated as
      * the -L_v W of jac plus W / mus at each interaction's voxel */
     int64_t *jac_mus;
+    /* per-photon path records (NULL: none), photon first + i at i:
+     * path_len[i * ntissue + t], path_int[i * ntissue + t] (path length and
+     * interactions in tissue t), path_w[i] (R exit weight, 0 when the photon
+     * left by another way), path_r[i] (exit radius) */
+    float *path_len;
+    int32_t *path_int;
+    float *path_w;
+    float *path_r;
 } orc_adjoint_out;
 
 int orc_adjoint(const orc_params *p, int64_t first, int64_t count, orc_tallies *t, orc_adjoint_out *out,

@@ -54,7 +54,9 @@ _lib = None
 
 class AdjointOut(C.Structure):
     _fields_ = [("jac_origin", C.c_float * 3), ("jac_size", C.c_float * 3), ("jac_n", C.c_int32 * 3),
-                ("signals", _lp), ("grad", _lp), ("jac", _lp), ("ndetected", C.c_int64), ("jac_mus", _lp)]
+                ("signals", _lp), ("grad", _lp), ("jac", _lp), ("ndetected", C.c_int64), ("jac_mus", _lp),
+                ("path_len", C.POINTER(C.c_float)), ("path_int", C.POINTER(C.c_int32)),
+                ("path_w", C.POINTER(C.c_float)), ("path_r", C.POINTER(C.c_float))]
 
 
 def lib():
@@ -287,11 +289,13 @@ class Run:
         assert rc == 0
         return ex, tr, tn
 
-    def adjoint(self, first, count, jac=None, nthreads=1, tallies=False, jac_mus=False):
+    def adjoint(self, first, count, jac=None, nthreads=1, tallies=False, jac_mus=False, paths=False):
         """orc_adjoint: (signals (ndet,), grad (ndet, ntissue, 2), jac
         (ndet, nvox) or None, pairs), 2^32 fixed point; jac = (origin,
         size, n) of the voxel grid or None.  With jac_mus, the per-voxel mus
-        Jacobian (ndet, nvox) is appended as a fifth element."""
+        Jacobian (ndet, nvox) is appended as a fifth element; with paths,
+        the per-photon path records dict(length (count, ntissue),
+        interactions (count, ntissue), w (count,), radius (count,)) last."""
         nd = self.P.ndet
         nt = self.P.ntissue
         A = AdjointOut()
@@ -308,6 +312,13 @@ class Run:
         A.jac = lptr(J) if J is not None else None
         Js = np.zeros_like(J) if (jac_mus and J is not None) else None
         A.jac_mus = lptr(Js) if Js is not None else None
+        if paths:
+            P = dict(length=np.zeros((count, nt), np.float32), interactions=np.zeros((count, nt), np.int32),
+                     w=np.zeros(count, np.float32), radius=np.zeros(count, np.float32))
+            A.path_len = P["length"].ctypes.data_as(C.POINTER(C.c_float))
+            A.path_int = P["interactions"].ctypes.data_as(C.POINTER(C.c_int32))
+            A.path_w = P["w"].ctypes.data_as(C.POINTER(C.c_float))
+            A.path_r = P["radius"].ctypes.data_as(C.POINTER(C.c_float))
         rc = lib().orc_adjoint(C.byref(self.P), first, count, C.byref(self.T) if tallies else None, C.byref(A),
                                nthreads)
         assert rc == 0
@@ -315,6 +326,8 @@ class Run:
                J.reshape(max(nd, 1), -1)[:nd] if J is not None else None, A.ndetected)
         if jac_mus:
             out += (Js.reshape(max(nd, 1), -1)[:nd] if Js is not None else None,)
+        if paths:
+            out += (P,)
         return out
 
     def tallies(self):

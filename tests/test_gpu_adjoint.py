@@ -46,17 +46,27 @@ def test_adjoint_bit_exact_vs_oracle(name, n):
     ctx.set_scene(sc, seed=21)
     ctx.set_detectors(lo, hi)
     ctx.set_jacobian_grid(*grid)
-    got = ctx.adjoint(0, n, jac=True, jac_mus=True)
+    got = ctx.adjoint(0, n, jac=True, jac_mus=True, paths=True)
     st = ctx.stats()
     ctx.close()
     run = O.Run(sc, seed=21, tallies={"det_rlo": lo, "det_rhi": hi})
-    sig, grad, jac, pairs, jac_s = run.adjoint(0, n, jac=grid, nthreads=8, jac_mus=True)
+    sig, grad, jac, pairs, jac_s, P = run.adjoint(0, n, jac=grid, nthreads=8, jac_mus=True, paths=True)
     assert st.replay_mismatch == 0 and st.replayed == pairs == got["pairs"] > 0
     assert np.array_equal(got["signals"], sig)
     assert np.array_equal(got["grad"], grad)
     assert np.array_equal(got["jac"].reshape(len(lo), -1), jac)
     assert np.array_equal(got["jac_mus"].reshape(len(lo), -1), jac_s)
     assert np.any(jac_s != jac)
+    # the per-pair path records, sorted by (photon, detector), against the
+    # oracle's per-photon records
+    G = got["paths"]
+    want = [(i, d) for i in range(n) for d in range(len(lo))
+            if P["w"][i] != 0 and lo[d] <= P["radius"][i] < hi[d]]
+    assert [(int(a), int(b)) for a, b in zip(G["photon"], G["det"])] == want
+    ids = G["photon"]
+    assert np.array_equal(G["w"], P["w"][ids])
+    assert np.array_equal(G["length"], P["length"][ids])
+    assert np.array_equal(G["interactions"], P["interactions"][ids].astype(np.uint32))
 
 
 def _bench_ctx():
@@ -156,6 +166,61 @@ def test_adjoint_outputs_are_cleared_when_nothing_is_detected():
         st = ctx.stats()
         assert r["pairs"] == 0 and st.replay_mismatch == 0
         assert not np.any(r["signals"]) and not np.any(r["grad"]) and not np.any(r["jac"])
+    ctx.close()
+
+
+def _reweighted(P, nm0, nm1, ndet, count):
+    """Signals per photon and their variances at optics nm1 from the path
+    records P of photons that ran at nm0 (perturbation Monte Carlo)."""
+    nm0, nm1 = nm0.astype(np.float64), nm1.astype(np.float64)
+    ok = nm0[:, 2] > 0
+    lr = P["interactions"] @ np.where(ok, np.log(np.where(ok, nm1[:, 2] / np.where(ok, nm0[:, 2], 1), 1)), 0)
+    lr -= P["length"].astype(np.float64) @ ((nm1[:, 1] - nm0[:, 1]) + (nm1[:, 2] - nm0[:, 2]))
+    w = P["w"].astype(np.float64) * np.exp(lr)
+    s1 = np.bincount(P["det"], w, ndet) / count
+    s2 = np.bincount(P["det"], w * w, ndet) / count
+    return s1, (s2 - s1 * s1) / count, np.bincount(P["det"], minlength=ndet), w
+
+
+def test_reweighted_paths_match_runs_at_other_optics():
+    # the path records of photons run at one set of optics, reweighted to
+    # another (fetal SpO2 0.50 -> 0.30 and maternal 0.97 -> 0.90 at 660 nm,
+    # and a 1 % change of the maternal fat's mus), agree with the signals
+    # of independent photons run at the other optics within 4 standard
+    # errors at every detector with 100 or more pairs: the estimator is
+    # unbiased (roulette and Fresnel decisions do not depend on the optics).
+    # A mus change spreads the log weights by about sqrt(n) |ln(mus'/mus)|
+    # (n: interactions in the tissue, hundreds in the fat): at 10 % the
+    # weights are heavy-tailed and 4e6 photons no longer sample the tail,
+    # so the sample mean falls short and its variance estimate with it.
+    from lightpropagation_amd import scenes as S
+    ctx, sc = _bench_ctx()
+    N = 4_000_000
+    nm0 = S.mf_optics(660.0, 0.97, 0.50)
+    others = [S.mf_optics(660.0, 0.97, 0.30), S.mf_optics(660.0, 0.90, 0.50)]
+    fat = nm0.copy()
+    fat[2, 2] *= 1.01
+    others.append(fat)
+    ctx.set_materials(nm0, sc["fetal_id"])
+    A = ctx.adjoint(0, N, paths=True)
+    nd = len(A["signals"])
+    s0, _, _, w0 = _reweighted(A["paths"], nm0, nm0, nd, N)
+    assert np.allclose(s0, A["signals"] / FX / N, rtol=1e-5, atol=0)  # the identity at the records' optics
+    for k, nm1 in enumerate(others):
+        ctx.set_materials(nm1, sc["fetal_id"])
+        B = ctx.adjoint((k + 1) * N, N, paths=True)
+        sa, va, na, wa = _reweighted(A["paths"], nm0, nm1, nd, N)
+        sb, vb, nb, _ = _reweighted(B["paths"], nm1, nm1, nd, N)
+        use = (na >= 100) & (nb >= 100)
+        z = (sa - sb) / np.sqrt(va + vb)
+        print("optics", k, "detectors", int(use.sum()), "z", np.round(z[use], 2), flush=True)
+        assert use.sum() >= 4 and np.all(np.abs(z[use]) < 4.0), z
+        # the change itself is seen: the reweighted signals differ from the
+        # records' own by more than 4 standard errors of that paired difference
+        dd = wa - w0
+        m1 = np.bincount(A["paths"]["det"], dd, nd) / N
+        vd = (np.bincount(A["paths"]["det"], dd * dd, nd) / N - m1 * m1) / N
+        assert np.any(np.abs(sa[use] - s0[use]) > 4 * np.sqrt(vd[use]))
     ctx.close()
 
 

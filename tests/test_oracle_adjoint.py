@@ -185,3 +185,85 @@ def test_voxel_mus_gradient_on_traces():
         tissue = grad[d, :, 1].sum() / FX
         voxels = jac_s[d].sum() / FX
         assert abs(voxels - tissue) <= 1e-5 * abs(tissue) + 1e-5, (d, voxels, tissue)
+
+
+def test_path_records_are_the_traced_paths():
+    # per-photon path records (the perturbation Monte Carlo inputs): the
+    # length per tissue from the traced positions, the interactions per
+    # tissue from the event kinds, and the gradients are their sums
+    sc, run = _c0_run(seed=17)
+    N = 400
+    nt = len(sc["nmuasg"])
+    sig, grad, _, pairs, P = run.adjoint(0, N, paths=True)
+    ex, tr, tn = run.run(0, N, exit_rec=True, trace_max=20000)
+    lo, hi = [0.0, 0.3, 1.0], [0.3, 1.0, 3.0]
+    assert np.array_equal(P["w"], ex[:, 3]) and np.array_equal(P["radius"], ex[:, 7])
+    p0 = np.asarray(sc["pstart"], np.float64)
+    want_g = np.zeros((3, nt, 2), np.int64)
+    npairs = 0
+    for i in range(N):
+        L = np.zeros(nt)
+        n = np.zeros(nt, np.int32)
+        prev, tissue = p0, 0
+        for k in range(tn[i]):
+            rec = tr[i, k]
+            pos = rec[4:7].astype(np.float64)
+            L[tissue] += np.linalg.norm(pos - prev)
+            if rec[0] == 0:
+                n[tissue] += 1
+            prev, tissue = pos, int(rec[2])
+        W = P["w"][i]
+        if W == 0:
+            continue
+        assert np.array_equal(P["interactions"][i], n), i
+        assert np.allclose(P["length"][i], L, rtol=2e-4, atol=1e-5), i
+        for d in range(3):
+            if lo[d] <= P["radius"][i] < hi[d]:
+                npairs += 1
+                for t in range(nt):
+                    Lt, nt_ = P["length"][i, t], P["interactions"][i, t]
+                    if Lt == 0 and nt_ == 0:
+                        continue
+                    mus = np.float32(sc["nmuasg"][t][2])
+                    gm = np.float32(-(Lt * W))
+                    gs = np.float32(np.float32(np.float32(nt_) / mus) - Lt) * W if mus != 0 else gm
+                    want_g[d, t, 0] += int(np.float32(gm) * np.float32(FX))
+                    want_g[d, t, 1] += int(np.float32(gs) * np.float32(FX))
+    assert npairs == pairs > 0
+    assert np.array_equal(grad, want_g)
+
+
+def test_path_records_reweight_to_other_albedos():
+    # optics with the same mu_t = mua + mus in every layer give every photon
+    # the same path (same step lengths, same draws) and change only the
+    # albedo of each interaction, so a photon's weight at the new optics is
+    # its weight at the old times prod_t (mus'_t / mus_t)^n_t: the records'
+    # interaction counts per tissue, and nothing else, decide it
+    sc, run = _c0_run(seed=19)
+    N = 600
+    _, _, _, _, P = run.adjoint(0, N, paths=True)
+    nm = np.array(sc["nmuasg"], np.float32)
+    nm2 = nm.copy()
+    f = np.float32
+    for t in range(1, len(nm)):
+        mus2 = f(nm[t, 2] - f(0.25))
+        mua2 = f(f(nm[t, 1] + nm[t, 2]) - mus2)
+        assert f(mua2 + mus2) == f(nm[t, 1] + nm[t, 2])
+        nm2[t, 1], nm2[t, 2] = mua2, mus2
+    sc2 = dict(sc, nmuasg=nm2)
+    run2 = O.Run(sc2, seed=19, tallies={"det_rlo": [0.0, 0.3, 1.0], "det_rhi": [0.3, 1.0, 3.0]})
+    ex2, _, _ = run2.run(0, N, exit_rec=True)
+    ratio = np.ones(len(nm))
+    ratio[1:] = nm2[1:, 2].astype(np.float64) / nm[1:, 2]
+    same = (P["w"] > 0) & (ex2[:, 7] == P["radius"])  # same exit: no roulette decided otherwise
+    assert same.sum() > 0.8 * (P["w"] > 0).sum() > 100
+    pred = P["w"].astype(np.float64) * np.prod(ratio[None, :] ** P["interactions"], axis=1)
+    assert np.allclose(ex2[same, 3], pred[same], rtol=2e-4, atol=0)
+    # one interaction booked to the wrong tissue moves the prediction by
+    # about 0.1 %: the check sees it
+    bad = P["interactions"].copy()
+    k = np.argmax(same & (bad[:, 1] > 0) & (bad[:, 2] > 0))
+    bad[k, 1] -= 1
+    bad[k, 2] += 1
+    wrong = P["w"][k] * np.prod(ratio ** bad[k])
+    assert abs(wrong - ex2[k, 3]) > 5e-4 * ex2[k, 3]

@@ -203,3 +203,99 @@ def test_plain_adam_stalls_in_the_valley():
     meas = [sim.adjoint(sim.optics(o.wavelength, 0.97, 0.50), 0, 1)["signals"] for o in opts]
     x, _ = TA.fit_spo2(sim, opts, meas, init=(0.85, 0.7), steps=50, lr=0.05, lr_end=0.001, rewhiten=None)
     assert np.abs(x - [0.97, 0.50]).max() > 0.05
+
+
+def _records(seed=3, P=5000, nt=5, nd=3):
+    rng = np.random.default_rng(seed)
+    nm = np.zeros((nt, 4), np.float32)
+    nm[:, 0] = 1.4
+    nm[:, 1] = rng.uniform(0.05, 0.5, nt)
+    nm[:, 2] = rng.uniform(10, 100, nt)
+    nm[0, 1:3] = 0.0  # an ambient start tissue: no interactions, mus 0
+    L = rng.exponential(0.5, (P, nt)).astype(np.float32)
+    L[:, 0] = 0.002
+    n = rng.poisson(L * nm[:, 2]).astype(np.uint32)
+    rec = dict(det=rng.integers(0, nd, P).astype(np.int32), w=rng.uniform(0.01, 0.5, P).astype(np.float32),
+               length=L, interactions=n)
+    return TA.PathRecords(rec, nm, 20 * P, nd), nm
+
+
+def test_reweighted_signals_and_their_gradient():
+    paths, nm = _records()
+    mua0 = torch.tensor(nm[:, 1], dtype=torch.float64)
+    mus0 = torch.tensor(nm[:, 2], dtype=torch.float64)
+    # at the records' own optics: the mean weight per photon, and the
+    # path-length estimator's gradient
+    mua, mus = mua0.clone().requires_grad_(True), mus0.clone().requires_grad_(True)
+    S = paths.signals(mua, mus)
+    want = np.bincount(paths.det.numpy(), paths.w.numpy(), 3) / paths.count
+    assert np.allclose(S.detach().numpy(), want, rtol=1e-12)
+    S[1].backward()
+    sel = (paths.det == 1).numpy()
+    w, L, n = paths.w.numpy()[sel], paths.L.numpy()[sel], paths.n.numpy()[sel]
+    assert np.allclose(mua.grad.numpy(), -(w @ L) / paths.count, rtol=1e-12)
+    gs = ((w @ n) / np.where(nm[:, 2] > 0, nm[:, 2], 1) - w @ L) / paths.count
+    assert np.allclose(mus.grad.numpy()[1:], gs[1:], rtol=1e-10)
+    # elsewhere: the backward pass is the derivative of the forward one
+    f = lambda a, s: paths.signals(a, s)  # noqa: E731
+    a1 = (mua0 * 1.3).requires_grad_(True)
+    s1 = (mus0 * 0.9).requires_grad_(True)
+    assert torch.autograd.gradcheck(f, (a1, s1), eps=1e-6, atol=1e-9, rtol=1e-6)
+    # reweighting to other optics multiplies each weight by the density ratio
+    w1 = paths.weights(a1.detach(), s1.detach()).numpy()
+    ratio = np.where(nm[:, 2] > 0, 0.9, 1.0)
+    r = np.prod(ratio ** paths.n.numpy(), 1) * np.exp(-paths.L.numpy() @ (a1.detach().numpy() - nm[:, 1] +
+                                                                         s1.detach().numpy() - nm[:, 2]))
+    assert np.allclose(w1, paths.w.numpy() * r, rtol=1e-10)
+
+
+def test_relative_variance_of_the_means():
+    paths, _ = _records(P=20000)
+    rv = paths.relative_variance().numpy()
+    w, d, N = paths.w.numpy(), paths.det.numpy(), paths.count
+    for k in range(3):
+        x = np.zeros(N)
+        x[:int((d == k).sum())] = w[d == k]  # the photons of detector k, the rest 0
+        assert np.isclose(rv[k], x.var() / N / x.mean() ** 2, rtol=1e-9)
+
+
+class PathSim(OpticsSim):
+    """Path records of a Beer-Lambert stand-in: detector d's pairs all have
+    path lengths C_d. (so the reweighted signals are exactly exp(-C mua))
+    and weights that scatter about that value; the measured signals are the
+    exact ones."""
+
+    def __init__(self, pairs=4000):
+        self.bl = BeerLambertSim()
+        self.pairs = pairs
+        self.nmuasg = self.bl.nmuasg
+        self.ndet = 8
+
+    def adjoint(self, nm, first, count, jac=False, paths=False):
+        C = self.bl.C
+        rng = np.random.default_rng(first + 7)
+        det = np.repeat(np.arange(8), self.pairs).astype(np.int32)
+        S = np.exp(-C @ nm[:, 1].astype(np.float64))
+        # weights scattered about the Beer-Lambert value, normalised so the
+        # mean per photon is exactly S at the optics the records ran at
+        u = rng.uniform(0.5, 1.5, det.size)
+        for d in range(8):
+            u[det == d] /= u[det == d].mean()
+        w = (S[det] * u * count / self.pairs).astype(np.float64)
+        rec = dict(det=det, w=w, length=C[det].astype(np.float32), interactions=np.zeros((det.size, len(nm)), np.uint32))
+        return dict(signals=S, paths=rec)
+
+
+@pytest.mark.parametrize("init", [(0.85, 0.7), (0.6, 0.2), (0.99, 0.9)])
+def test_fit_on_path_records_recovers_the_saturations(init):
+    sim = PathSim()
+    opts = [TA.SpO2Optics(sim, wl) for wl in (660.0, 940.0)]
+    meas = [sim.bl.adjoint(sim.optics(o.wavelength, 0.97, 0.50), 0, 1)["signals"] for o in opts]
+    x, info = TA.fit_spo2_paths(sim, opts, meas, init=init, count=100_000, rounds=2, steps=150, min_pairs=100)
+    # float32 path lengths: the reweighted model is Beer-Lambert to 1e-7
+    assert np.allclose(x, [0.97, 0.50], atol=1e-3), x
+    assert len(info["rounds"]) == 2 and info["rounds"][-1]["loss_last"] < 1e-6
+    assert info["dof"] == 16 - 2 and info["cov"].shape == (2, 2) and np.all(np.diag(info["cov"]) > 0)
+    # the maternal saturation known: only the fetal one moves
+    x, info = TA.fit_spo2_paths(sim, opts, meas, init=(0.97, init[1]), count=100_000, rounds=2, free=(False, True))
+    assert x[0] == 0.97 and abs(x[1] - 0.50) < 1e-3 and info["cov"][0, 0] == 0

@@ -42,7 +42,7 @@ METRIC = "photons/sec on fetal mesh (1 GPU); adjoint vs finite-diff gradient tim
 # triangle test (52 + 3) plus ceil(log2 T) box tests of 20.
 F_INT, F_BND, F_TRI, F_BOX = 225.0, 125.0, 55.0, 20.0
 FP32_PEAK_TFLOPS = 157.3
-PROFILE = os.path.join(ROOT, "profiles", "transport_kernel_maternal_fetal_202k.md")
+PROFILE = os.path.join(ROOT, "profiles", "transport_kernel_maternal_fetal_196k.md")
 
 
 def traffic_per_photon():
@@ -530,7 +530,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": (tpp * args.photons) if tpp is not None else None,
-                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)",
+                     "traffic_unit": "bytes per 660 nm launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)",
                      "work": "algorithmic fp32 per step: interaction %g + boundary %g, search %g (T=%d)" %
                              (F_INT, F_BND, search, T)},
         "counters": counters,

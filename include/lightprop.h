@@ -224,7 +224,10 @@ int lp_get_adjoint_paths(lp_context *ctx, int64_t npairs, int64_t *photon, int32
                          uint32_t *interactions);
 
 /* Evaluates the kernel's math core element-wise on the device (op 0 sqrt,
- * 1 divide, 2 log, 3 cos, 4 acos, 5 multiply, 6 add) over n (x, y) pairs.
+ * 1 divide, 2 log, 3 cos, 4 acos, 5 multiply, 6 add, 7 the square root of
+ * the scattering path, exact for 0 and arguments of at least 2^-96, 8 the
+ * reciprocal of its normalize, exact on [2^-60, 2^60], 9 the step-length
+ * log, for draws in [2^-126, 1]) over n (x, y) pairs.
  * No reference counterpart (the reference calls OpenCL built-ins): it lets
  * the tests compare the device math with the oracle's bit for bit. */
 int lp_debug_math(lp_context *ctx, int32_t op, const float *xy, float *out, int64_t n);

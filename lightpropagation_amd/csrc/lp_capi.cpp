@@ -398,6 +398,7 @@ void fill_args(lp_context* c, KArgs& a)
     std::memcpy(a.vox_size, c->vox_size, sizeof a.vox_size);
     for (int k = 0; k < 3; ++k) a.vox_inv[k] = 1.0f / c->vox_size[k];
     std::memcpy(a.vox_n, c->vox_n, sizeof a.vox_n);
+    for (int k = 0; k < 3; ++k) a.vox_nf[k] = (float)c->vox_n[k];
     a.rad_dr = c->rad_dr;
     a.rad_n = c->rad_n;
     a.ndet = (int32_t)c->det_rlo.size();
@@ -1172,7 +1173,11 @@ int lp_trace(lp_context* c, int64_t first, int32_t count, int32_t max_rec, float
     a.trace_n = (int32_t*)tn.p;
     a.trace_max = max_rec;
     if (c->record_exit) {
-        if ((rc = dev_alloc(c, c->d_exit, (size_t)count * 8 * sizeof(float)))) return rc;
+        if ((rc = dev_alloc(c, c->d_exit, (size_t)count * 8 * sizeof(float)))) {
+            dev_free(tr);
+            dev_free(tn);
+            return rc;
+        }
         a.exit_rec = (float*)c->d_exit.p;
         c->exit_count = count;
     }

@@ -173,6 +173,7 @@ struct KArgs {
     unsigned long long *sft, *rtd, *arz, *phirz, *region, *vox, *rad, *det, *misc;
     float vox_origin[3], vox_size[3], vox_inv[3];  // vox_inv = 1 / vox_size in float
     int32_t vox_n[3];
+    float vox_nf[3];  // vox_n as floats
     float rad_dr;
     int32_t rad_n;
     int32_t ndet;

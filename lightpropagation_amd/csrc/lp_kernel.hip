@@ -99,6 +99,24 @@ struct PhiloxState {
         left--;
         return (float)(u >> 8) * 5.96046448e-08f;
     }
+    // the next two words in stream order, with one shift of the buffer
+    __device__ __forceinline__ void next2(uint32_t k0, uint32_t k1, float& x0, float& x1)
+    {
+        if (left < 2) {  // more draws in one iteration than top_up provides (rare)
+            x0 = next(k0, k1);
+            x1 = next(k0, k1);
+            return;
+        }
+        const uint32_t u0 = b0, u1 = b1;
+        b0 = b2;
+        b1 = b3;
+        b2 = b4;
+        b3 = b5;
+        b4 = b6;
+        left -= 2;
+        x0 = (float)(u0 >> 8) * 5.96046448e-08f;
+        x1 = (float)(u1 >> 8) * 5.96046448e-08f;
+    }
 };
 
 // PSNR::InitRand3 (src/PSNR/PSNR.cpp:48-80) in int32 (all values < 2^31),
@@ -172,6 +190,16 @@ struct Rng {
         if (COUNT) ndraw++;
         if (RNG == 0) return ph.next(k0, k1);
         return ran3f_next(s, tid, inext, inextp);
+    }
+    __device__ __forceinline__ void next2(float& x0, float& x1)
+    {
+        if (COUNT) ndraw += 2;
+        if (RNG == 0) {
+            ph.next2(k0, k1, x0, x1);
+        } else {
+            x0 = ran3f_next(s, tid, inext, inextp);
+            x1 = ran3f_next(s, tid, inext, inextp);
+        }
     }
 };
 
@@ -394,22 +422,32 @@ __device__ __forceinline__ Geom geom_of(const KArgs& a, const float4* lmat, int3
     return g;
 }
 
+// Per-wavefront LDS rows of the near-face list query: one slot per lane,
+// holding that lane's results while the pairs of its list are merged in.
+struct NearRed {
+    unsigned long long key[64];  // (t bits, face id) of the nearest hit
+    unsigned long long d1[64];   // (squared distance bits, list record) of the nearest listed face
+    uint32_t d2[64];             // bits of the second-smallest squared distance
+    int32_t med[64];             // packed tissues of the nearest hit
+};
+
 // Near-face list query, wave-cooperative, for the lanes with `part` set: the
 // (photon, listed face) pairs of all those lanes are numbered by a prefix sum
 // over their list lengths and tested 64 at a time, one pair per lane, so a
 // pass costs one face test whatever the lengths of the individual lists.
-// Each pair's outcome is merged into its photon's result by a segmented
-// reduction (pairs of one photon are consecutive): the nearest hit is the
-// lexicographic minimum of (t, face id) over the hits with t <= tmax (the
-// nearest face and tie rule of find_face) and, when want, the two smallest
-// distances from p to the listed faces with the record of the smaller;
-// unlisted = a lower bound on the distance to every face not in the list.
+// Each pair's outcome is merged into its photon's slot of R by LDS atomic
+// minima: the nearest hit is the lexicographic minimum of (t, face id) over
+// the hits with t <= tmax (the nearest face and tie rule of find_face) and,
+// when want, the two smallest distances from p to the listed faces with the
+// record of the smaller (a value that loses a minimum, or is displaced from
+// it, is a candidate for the second smallest); unlisted = a lower bound on
+// the distance to every face not in the list.
 // The list is that of the cell holding cq (a point on the ray: p, or where
 // a march along it has got to); rays and distances are from p.
 // Must be called by the whole wavefront.
-__device__ __forceinline__ void near_batch(const KArgs& a, bool part, uint32_t coff, f3 cq, f3 p, f3 v, int32_t ref,
-                                        float tmax, bool want, Hit& h, float& d1sq, float& d2sq, int32_t& i1,
-                                        float& unlisted, uint32_t& pairs)
+__device__ __forceinline__ void near_batch(const KArgs& a, NearRed& R, bool part, uint32_t coff, f3 cq, f3 p, f3 v,
+                                           int32_t ref, float tmax, bool want, Hit& h, float& d1sq, float& d2sq,
+                                           int32_t& i1, float& unlisted, uint32_t& pairs)
 {
     const int lane = threadIdx.x & 63;
     h.t = tmax;
@@ -438,8 +476,10 @@ __device__ __forceinline__ void near_batch(const KArgs& a, bool part, uint32_t c
     const uint32_t excl = incl - n;
     const uint32_t total = __shfl(incl, 63);
     pairs = total;
-    uint64_t best = ~0ull;  // (t bits, face id) of the nearest hit so far
-    int32_t best_med = 0;
+    R.key[lane] = ~0ull;
+    R.d1[lane] = ~0ull;
+    R.d2[lane] = 0x7f800000u;  // +inf
+    R.med[lane] = 0;
     const float4* tr = reinterpret_cast<const float4*>(a.near_tris);
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t g = base + lane;
@@ -458,11 +498,15 @@ __device__ __forceinline__ void near_batch(const KArgs& a, bool part, uint32_t c
         const uint32_t k = __shfl(b, j) + (g - __shfl(excl, j));
         uint64_t key = ~0ull;
         int32_t med = 0;
-        float dd = __builtin_inff();
         if (g < total) {
             const float4 A = tr[3 * (int64_t)k], B = tr[3 * (int64_t)k + 1], C = tr[3 * (int64_t)k + 2];
             const f3 M = mk(A.x, A.y, A.z), E = mk(B.x, B.y, B.z), F = mk(C.x, C.y, C.z);
-            if (owant) dd = point_tri_dist2(op, M, E, F);
+            if (owant) {
+                const float dd = point_tri_dist2(op, M, E, F);
+                const unsigned long long dk = ((unsigned long long)__float_as_uint(dd) << 32) | k;
+                const unsigned long long old = atomicMin(&R.d1[j], dk);
+                atomicMin(&R.d2[j], (uint32_t)((old > dk ? old : dk) >> 32));
+            }
             const int32_t id = __float_as_int(A.w);
             const f3 G = sub(op, M);
             const f3 GxE = cross(G, E), vxF = cross(ov, F);
@@ -479,66 +523,32 @@ __device__ __forceinline__ void near_batch(const KArgs& a, bool part, uint32_t c
                 if (t > 0.0f && kk >= 0.0f && l >= 0.0f && (kk + l) < 1.0f && t < 10000.0f && t <= otmax) {
                     key = ((uint64_t)__float_as_uint(t) << 32) | (uint32_t)id;
                     med = __float_as_int(B.w);
+                    atomicMin(&R.key[j], (unsigned long long)key);
                 }
             }
         }
-        // segmented reduction towards the first pair of each photon in this pass
-        float e1 = dd, e2 = __builtin_inff();
-        int32_t ek = (int32_t)k;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int src = lane + off < 64 ? lane + off : lane;
-            const int sj = __shfl(j, src);
-            const uint32_t klo = __shfl((uint32_t)key, src), khi = __shfl((uint32_t)(key >> 32), src);
-            const int32_t smed = __shfl(med, src);
-            const float s1 = __shfl(e1, src), s2 = __shfl(e2, src);
-            const int32_t sk = __shfl(ek, src);
-            if (src != lane && sj == j) {
-                const uint64_t skey = ((uint64_t)khi << 32) | klo;
-                if (skey < key) {
-                    key = skey;
-                    med = smed;
-                }
-                if (s1 < e1) {
-                    e2 = fminf(e1, s2);
-                    e1 = s1;
-                    ek = sk;
-                } else {
-                    e2 = fminf(e2, s1);
-                }
-            }
+        // every lane's minimum of this pass is in place (a wavefront's LDS
+        // operations complete in order); the pair holding its owner's
+        // current nearest hit writes that hit's tissues
+        __builtin_amdgcn_wave_barrier();
+        if (key != ~0ull && R.key[j] == key) R.med[j] = med;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (part) {
+        const unsigned long long best = R.key[lane];
+        if (best != ~0ull) {
+            h.t = __uint_as_float((uint32_t)(best >> 32));
+            h.id = (int32_t)(uint32_t)best;
+            h.med = R.med[lane];
         }
-        // each owner takes the result at its first pair in this pass
-        const bool mine = n > 0 && excl < base + 64 && excl + n > base;
-        const int head = mine ? (int)((excl > base ? excl : base) - base) : lane;
-        const uint32_t rlo = __shfl((uint32_t)key, head), rhi = __shfl((uint32_t)(key >> 32), head);
-        const int32_t rmed = __shfl(med, head);
-        const float r1 = __shfl(e1, head), r2 = __shfl(e2, head);
-        const int32_t rk = __shfl(ek, head);
-        if (mine) {
-            const uint64_t rkey = ((uint64_t)rhi << 32) | rlo;
-            if (rkey < best) {
-                best = rkey;
-                best_med = rmed;
-            }
-            if (r1 < d1sq) {
-                d2sq = fminf(d1sq, r2);
-                d1sq = r1;
-                i1 = rk;
-            } else {
-                d2sq = fminf(d2sq, r1);
-            }
+        const unsigned long long d1k = R.d1[lane];
+        if (want && d1k != ~0ull) {
+            d1sq = __uint_as_float((uint32_t)(d1k >> 32));
+            i1 = (int32_t)(uint32_t)d1k;
+            d2sq = __uint_as_float(R.d2[lane]);
         }
     }
-    if (best != ~0ull) {
-        h.t = __uint_as_float((uint32_t)(best >> 32));
-        h.id = (int32_t)(uint32_t)best;
-        h.med = best_med;
-    }
-    if (!want) {
-        d1sq = d2sq = __builtin_inff();
-        i1 = -1;
-    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v)
@@ -681,6 +691,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
 #define a (*(const KArgs*)ap)
     extern __shared__ float4 smem[];
     __shared__ unsigned long long wcnt[kBlock / 64][kWc];
+    __shared__ NearRed nred[kBlock / 64];
     float4* lnodes = smem;                         // 2 * nlds
     float4* lmat = smem + 2 * a.nlds;              // lmat_n: per instance, ntissue materials + (rsp, 0, 0, 0)
     float* lalb = reinterpret_cast<float*>(lmat + a.lmat_n);  // lmat_n: mua / (mua + mus) of each row
@@ -713,11 +724,13 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
     __syncthreads();
     const int32_t ntp1 = a.ntissue + 1;  // material rows per instance
 
-    const f3 POI = mk(a.poi[0], a.poi[1], a.poi[2]);
-    const f3 eX = mk(a.ex[0], a.ex[1], a.ex[2]);
-    const f3 eY = mk(a.ey[0], a.ey[1], a.ey[2]);
-    const f3 eZ = mk(a.ez[0], a.ez[1], a.ez[2]);
-    const f3 eZr = mk(a.ez_raw[0], a.ez_raw[1], a.ez_raw[2]);
+    // the emitter frame, read from the argument segment where used (held in
+    // scalar registers for the whole loop it would spill to vector lanes)
+#define POI mk(a.poi[0], a.poi[1], a.poi[2])
+#define eX mk(a.ex[0], a.ex[1], a.ex[2])
+#define eY mk(a.ey[0], a.ey[1], a.ey[2])
+#define eZ mk(a.ez[0], a.ez[1], a.ez[2])
+#define eZr mk(a.ez_raw[0], a.ez_raw[1], a.ez_raw[2])
 
     Rng<RNG, TRACE> rng;
     rng.s = lran3;
@@ -803,8 +816,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             absorbed = ph.w * lalb[ph.inst * ntp1 + ph.curr];  // ph.w * (mua / (mua + mus))
             ph.w -= absorbed;
         }
-        float xt = rng.next();
-        float xp = rng.next();
+        float xt, xp;
+        rng.next2(xt, xp);
             {  // VScatter (.cl:597-655)
             const f3 v = ph.v;
             const float g = cm.w;
@@ -818,7 +831,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             } else if (v.x != 0.0f) {
                 e_rot = mk(0.0f, 1.0f, 0.0f);
             }
-            e_rot = normalize(e_rot);
+            e_rot = normalize_n(e_rot);  // squared norm >= 2
             float cos_t;
             float g1 = (1 - g * g) / (1 - g + 2 * g * xt);
             if (g != 0.0f) {
@@ -828,25 +841,27 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             } else {
                 cos_t = 2 * xt - 1;
             }
-            float sin_t = __builtin_sqrtf(1.0f - cos_t * cos_t);
+            float sin_t = sqrt_n(1.0f - cos_t * cos_t);
             float psi = 6.28318548f * xp;
             float cos_p = cosf_lp(psi);
-            float sin_p = __builtin_sqrtf(1.0f - cos_p * cos_p);
+            float sin_p = sqrt_n(1.0f - cos_p * cos_p);
             if (psi > 3.14159274f) sin_p *= -1;
             // Rotate (.cl:697-708) twice
             float d1 = dot(e_rot, v);
             f3 c1 = cross(e_rot, v);
             f3 c1n = cross(c1, e_rot);
-            f3 rt = normalize(add(add(mul(d1, e_rot), mul(cos_t, c1n)), mul(sin_t, c1)));
+            // rt and the new direction are sums of orthogonal parts of norm
+            // near 1 (e_rot is normal to v)
+            f3 rt = normalize_n(add(add(mul(d1, e_rot), mul(cos_t, c1n)), mul(sin_t, c1)));
             float d2 = dot(v, rt);
             f3 c2 = cross(v, rt);
             f3 c2n = cross(c2, v);
-            ph.v = normalize(add(add(mul(d2, v), mul(cos_p, c2n)), mul(sin_p, c2)));
+            ph.v = normalize_n(add(add(mul(d2, v), mul(cos_p, c2n)), mul(sin_p, c2)));
         }
             // absorption tallies at the new position (replaces RecordAbsorbtionRZ, .cl:254-271)
         const f3 d = sub(ph.p, POI);
         const f3 c = mk(dot(eX, d), dot(eY, d), dot(eZ, d));
-        unsigned long long q = fx32(absorbed);
+        unsigned long long q = fx32_nn(absorbed);
         reg_acc += q;
         if (a.arz) {
             float rho = __builtin_sqrtf(c.x * c.x + c.y * c.y) * sign_cl(c.x) / a.det_grid[3];
@@ -866,8 +881,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             float fz = (c.z - a.vox_origin[2]) * a.vox_inv[2];
             // the truncation lies in [0, n) exactly when x does (n < 2^24 is
             // exact in float): the host's int() binning with its bounds check
-            const bool ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)a.vox_n[0] &&
-                            fy < (float)a.vox_n[1] && fz < (float)a.vox_n[2];
+            const bool ok = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < a.vox_nf[0] && fy < a.vox_nf[1] &&
+                            fz < a.vox_nf[2];
             // consecutive deposits in one voxel are summed in a register
             // (value selects, so the accumulator stays in registers); fewer
             // than 2^31 voxels, so 32-bit indices
@@ -1225,6 +1240,17 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             xsft = mk(0.0f, 0.0f, 0.0f);
             if (alive) rng.top_up();
             bool act = alive && !search;
+            // the clearance of the near-grid cell holding p, loaded before the
+            // step length is drawn so that the load overlaps the draw; used
+            // when the free region cannot take the step
+            const bool pre = act && !a.no_skip && a.near_cells && a.clear_step >= 0.0f;
+            int32_t pre_c = -1;
+            uint32_t pre_bits = 0u;
+            if (pre) {
+                pre_c = near_cell(a, ph.p);
+                const uint32_t ci = (uint32_t)(pre_c >= 0 ? pre_c : 0) + geom_of<MG>(a, lmat, ph.inst).coff;
+                pre_bits = a.near_cells[ci].clear_bits;
+            }
             if (act) {
                 if (!have_step) {
                     const float4 cm = lmat[ph.inst * ntp1 + ph.curr];
@@ -1235,7 +1261,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                             do {
                                 x = rng.next();
                             } while (x <= 0.0f);
-                            stepsize = -logf_lp(x) * a.factor / (cm.y + cm.z);
+                            stepsize = -logf_u01(x) * a.factor / (cm.y + cm.z);
                         } else {
                             stepsize = ph.unfin / (cm.y + cm.z);
                             ph.unfin = 0.0f;
@@ -1252,10 +1278,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                     // else the ball of the near-grid cell's clearance around p:
                     // no face lies within the clearance of any point the cell
                     // lookup places in the cell
-                    const int32_t c = near_cell(a, ph.p);
-                    if (c >= 0) {
-                        const float cl =
-                            __uint_as_float(a.near_cells[(uint32_t)c + geom_of<MG>(a, lmat, ph.inst).coff].clear_bits);
+                    if (pre_c >= 0) {
+                        const float cl = __uint_as_float(pre_bits);
                         const float rad = fmaxf(fmaxf(__builtin_fabsf(ph.p.x), __builtin_fabsf(ph.p.y)),
                                                 __builtin_fabsf(ph.p.z));
                         const float sc = cl - (a.safe_margin + 1e-5f * (cl + rad));
@@ -1315,7 +1339,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                 }
                 Hit hl;
                 uint32_t pairs;
-                near_batch(a, part, coff, cq, ph.p, ph.v, ph.ref, wmax, want, hl, d1sq, d2sq, i1, unlisted, pairs);
+                near_batch(a, nred[tid >> 6], part, coff, cq, ph.p, ph.v, ph.ref, wmax, want, hl, d1sq, d2sq, i1, unlisted,
+                           pairs);
                 r_lst += pairs;
                 if (part && reach) {
                     if (hl.id >= 0) {  // the nearest face ahead
@@ -1512,6 +1537,11 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
         }
     }
 #undef a
+#undef POI
+#undef eX
+#undef eY
+#undef eZ
+#undef eZr
 }
 
 // ---------------------------------------------------------------------------
@@ -1623,6 +1653,9 @@ __global__ void debug_math_kernel(int op, const float* in, float* out, int64_t n
     case 3: r = cosf_lp(x); break;
     case 4: r = acosf_lp(x); break;
     case 5: r = x * y; break;
+    case 7: r = sqrt_n(x); break;
+    case 8: r = recip_n(x); break;
+    case 9: r = logf_u01(x); break;
     default: r = x + y; break;
     }
     out[i] = r;

@@ -81,6 +81,46 @@ def test_math_core_bit_exact(ctx):
                           np.array([O.acosf(v) for v in cs], np.float32).view(np.uint32))
 
 
+def test_scattering_sqrt_correctly_rounded(ctx):
+    # op 7 (the scattering path's square root) is correctly rounded on its
+    # domain: 0, and arguments from 2^-96 up (1 - c^2 of floats c in [-1, 1],
+    # squared norms near 1), including every float in [0.5, 4)
+    rng = np.random.default_rng(3)
+    c = rng.uniform(-1, 1, 200000).astype(np.float32)
+    x = np.concatenate([
+        (np.float32(1) - c * c).astype(np.float32),
+        np.exp(rng.uniform(-66, 88, 200000)).astype(np.float32),
+        np.arange(0x3f000000, 0x40800000, 7, dtype=np.uint32).view(np.float32),
+        np.array([0.0, 2.0 ** -96, 1.0, 2.0 ** -25, 3.4e38, np.inf], np.float32),
+    ])
+    x = x[(x == 0) | (x >= np.float32(2.0 ** -96))]
+    got = ctx.debug_math(7, x)
+    assert np.array_equal(got.view(np.uint32), np.sqrt(x).view(np.uint32))
+
+
+def test_normalize_reciprocal_correctly_rounded(ctx):
+    # op 8 (1 / y of normalize on the scattering path) equals the correctly
+    # rounded 1 / y on y in [2^-60, 2^60], densely on [0.5, 4)
+    rng = np.random.default_rng(4)
+    y = np.concatenate([
+        np.exp2(rng.uniform(-60, 60, 300000)).astype(np.float32),
+        np.arange(0x3f000000, 0x40800000, 5, dtype=np.uint32).view(np.float32),
+        np.array([2.0 ** -60, 2.0 ** 60, 1.0, 0.75, 3.0], np.float32),
+    ])
+    got = ctx.debug_math(8, y)
+    assert np.array_equal(got.view(np.uint32), (np.float32(1) / y).view(np.uint32))
+
+
+def test_step_log_equals_general_log(ctx):
+    # op 9 (the step-length log, without special cases) equals op 2 on the
+    # draws it meets: Philox words (k 2^-24, k >= 1) and ran3 values r / 1e9
+    rng = np.random.default_rng(5)
+    k = np.concatenate([np.arange(1, 5000), rng.integers(1, 1 << 24, 200000)]).astype(np.float32)
+    r = np.concatenate([np.arange(1, 3000), rng.integers(1, 10 ** 9, 100000)]).astype(np.float32)
+    x = np.concatenate([k * np.float32(5.96046448e-08), r / np.float32(1e9)]).astype(np.float32)
+    assert np.array_equal(ctx.debug_math(9, x).view(np.uint32), ctx.debug_math(2, x).view(np.uint32))
+
+
 CASES = [("model1", 512), ("model2", 512), ("model6FB", 256), ("c0slab", 256), ("model4", 48), ("model3", 256),
          ("planar", 128), ("escape", 256), ("oblique", 256)]
 

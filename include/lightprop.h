@@ -275,7 +275,7 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
 #define LP_OPT_WAVES_PER_SIMD 7 /* register budget of the kernel variant: 1 (compiler's
                                    choice), 4 (default), 5 or 6 waves per SIMD     */
 #define LP_OPT_NEAR_R0 8        /* near-face lists: radius r0 (0 = off, default 0.07) */
-#define LP_OPT_NEAR_CELL 9      /* near-face lists: cell edge (0 = 2 r0)            */
+#define LP_OPT_NEAR_CELL 9      /* near-face lists: cell edge (0 = r0 / 0.7)        */
 #define LP_OPT_SEARCH_MIN 10    /* run a BVH search batch once this fraction of a
                                    wavefront's live lanes waits for one (0..1)     */
 #define LP_OPT_LIST_MIN 11      /* run a near-list search batch once this fraction

@@ -79,7 +79,7 @@ struct lp_context {
     Bvh bvh;
     SafetyGrid sg;
     NearGrid ng;
-    float near_r0 = 0.07f, near_cell = 0.0f;  // cell 0 = 2 r0
+    float near_r0 = 0.07f, near_cell = 0.0f;  // cell 0 = kNearCellPerR0 r0
     std::vector<int32_t> part_root;   // BVH root of each geometry
     std::vector<uint32_t> part_coff;  // near-grid cell offset of each geometry
     DevBuf d_near_cells, d_near_tris;
@@ -251,7 +251,7 @@ int prepare(lp_context* c)
         c->bvh_extent = 0.0f;
         for (float x : c->tri) c->bvh_extent = std::max(c->bvh_extent, std::fabs(x));
         int cells = c->safety_cells < 0 ? 0 : c->safety_cells;  // off unless asked: the free radius covers it
-        const float cell = c->near_cell > 0 ? c->near_cell : 2 * c->near_r0;
+        const float cell = c->near_cell > 0 ? c->near_cell : kNearCellPerR0 * c->near_r0;
         if (c->part_first.size() > 2) {
             if (cells > 0) return fail(c, LP_EINVAL, "the safety grid needs a single geometry");
             const int ng = (int)c->part_first.size() - 1;

@@ -443,10 +443,13 @@ struct NearRed {
 // it, is a candidate for the second smallest); unlisted = a lower bound on
 // the distance to every face not in the list.
 // The list is that of the cell holding cq (a point on the ray: p, or where
-// a march along it has got to); rays and distances are from p.
+// a march along it has got to); rays and distances are from p.  With at_p
+// (cq is p) the cell's record comes from the caller: cell pc (< 0: outside
+// the grid), list start pfirst, clearance bits pbits, next list start pnext.
 // Must be called by the whole wavefront.
 __device__ __forceinline__ void near_batch(const KArgs& a, NearRed& R, bool part, uint32_t coff, f3 cq, f3 p, f3 v,
-                                           int32_t ref, float tmax, bool want, Hit& h, float& d1sq, float& d2sq,
+                                           int32_t ref, float tmax, bool want, bool at_p, int32_t pc, uint32_t pfirst,
+                                           uint32_t pbits, uint32_t pnext, Hit& h, float& d1sq, float& d2sq,
                                            int32_t& i1, float& unlisted, uint32_t& pairs)
 {
     const int lane = threadIdx.x & 63;
@@ -457,7 +460,13 @@ __device__ __forceinline__ void near_batch(const KArgs& a, NearRed& R, bool part
     i1 = -1;
     unlisted = a.near_r0;  // the grid reaches r0 beyond every face
     uint32_t b = 0, n = 0;
-    if (part) {
+    if (part && at_p) {
+        if (pc >= 0) {
+            b = pfirst;
+            n = pnext - pfirst;
+            unlisted = fmaxf(a.near_r0, __uint_as_float(pbits));
+        }
+    } else if (part) {
         const int32_t c0 = near_cell(a, cq);
         if (c0 >= 0) {
             const uint32_t c = (uint32_t)c0 + coff;
@@ -467,14 +476,19 @@ __device__ __forceinline__ void near_batch(const KArgs& a, NearRed& R, bool part
             unlisted = fmaxf(a.near_r0, __uint_as_float(cr.clear_bits));
         }
     }
-    uint32_t incl = n;  // inclusive prefix sum of the list lengths
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = __shfl_up(incl, off);
-        if (lane >= off) incl += t;
-    }
-    const uint32_t excl = incl - n;
-    const uint32_t total = __shfl(incl, 63);
+    // inclusive prefix sum of the list lengths over the wavefront, by DPP:
+    // shifts by 1, 2, 4, 8 within each row of 16 lanes, then lane 15 of a row
+    // added to the next row and lane 31 to the upper two rows (lanes a
+    // shift reads from outside its row take 0)
+    int incl = (int)n;
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, false);  // row_shr:1
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, false);  // row_shr:2
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, false);  // row_shr:4
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, false);  // row_shr:8
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    const uint32_t excl = (uint32_t)incl - n;
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
     pairs = total;
     R.key[lane] = ~0ull;
     R.d1[lane] = ~0ull;
@@ -1230,6 +1244,11 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
         // that they are refilled.
         for (int it = 0;; ++it) {
             launder();
+            // the Philox key from the argument segment each iteration, so that
+            // its round keys are formed in scalar registers where used rather
+            // than held for the whole loop (and spilled to vector lanes)
+            rng.k0 = (uint32_t)a.seed;
+            rng.k1 = (uint32_t)(a.seed >> 32);
             bool go = false;     // this lane takes a step in this iteration
             bool freed = false;  // ... and the step was shown to stay in the free region
             bool skipped = false;  // ... with no face search
@@ -1240,16 +1259,19 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             xsft = mk(0.0f, 0.0f, 0.0f);
             if (alive) rng.top_up();
             bool act = alive && !search;
-            // the clearance of the near-grid cell holding p, loaded before the
-            // step length is drawn so that the load overlaps the draw; used
-            // when the free region cannot take the step
-            const bool pre = act && !a.no_skip && a.near_cells && a.clear_step >= 0.0f;
+            // the record of the near-grid cell holding p (list start, clearance,
+            // the next cell's list start), loaded before the step length is
+            // drawn so that the load overlaps the draw: the clearance check
+            // and a list query from p use it
             int32_t pre_c = -1;
-            uint32_t pre_bits = 0u;
-            if (pre) {
+            uint32_t pre_first = 0u, pre_bits = 0u, pre_next = 0u;
+            if (alive && a.near_cells) {
                 pre_c = near_cell(a, ph.p);
                 const uint32_t ci = (uint32_t)(pre_c >= 0 ? pre_c : 0) + geom_of<MG>(a, lmat, ph.inst).coff;
-                pre_bits = a.near_cells[ci].clear_bits;
+                const uint32_t* cw = reinterpret_cast<const uint32_t*>(a.near_cells) + 2 * (size_t)ci;
+                pre_first = cw[0];
+                pre_bits = cw[1];
+                pre_next = cw[2];
             }
             if (act) {
                 if (!have_step) {
@@ -1332,15 +1354,16 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                 const uint32_t coff = part ? geom_of<MG>(a, lmat, ph.inst).coff : 0u;
                 f3 cq = ph.p;
                 float wmax = reach ? a.near_r0 : stepsize;
-                if (part && reach && mq > 0.0f) {
+                const bool at_p = !(reach && mq > 0.0f);
+                if (part && !at_p) {
                     cq = add(ph.p, mul(mq, ph.v));
                     const float rq = fmaxf(fmaxf(__builtin_fabsf(cq.x), __builtin_fabsf(cq.y)), __builtin_fabsf(cq.z));
                     wmax = fminf(mq + (a.near_r0 - (a.safe_margin + 1e-5f * (a.near_r0 + rq))), tlim);
                 }
                 Hit hl;
                 uint32_t pairs;
-                near_batch(a, nred[tid >> 6], part, coff, cq, ph.p, ph.v, ph.ref, wmax, want, hl, d1sq, d2sq, i1, unlisted,
-                           pairs);
+                near_batch(a, nred[tid >> 6], part, coff, cq, ph.p, ph.v, ph.ref, wmax, want, at_p, pre_c, pre_first,
+                           pre_bits, pre_next, hl, d1sq, d2sq, i1, unlisted, pairs);
                 r_lst += pairs;
                 if (part && reach) {
                     if (hl.id >= 0) {  // the nearest face ahead

@@ -673,6 +673,10 @@ __device__ __forceinline__ void jac_dda(const KArgs& a, unsigned long long* jd, 
     }
 }
 
+// boundary events in a row after which a lane's searches no longer wait
+// for a full batch
+constexpr uint32_t kUrgentRun = 16;
+
 // per-wavefront event counters in LDS
 enum WaveCount {
     wPhot, wInt, wBnd, wEsc, wCap, wSkip, wDq, wNan, wLst, wIter, wLRounds, wBRounds, wArzOut, wVoxOut, wMismatch,
@@ -1333,9 +1337,14 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             const int nlist = __popcll(__ballot(search && by_list));
             const int nbvh = __popcll(__ballot(search && !by_list));
             const bool stuck = !__any(go);
+            // a lane in a run of boundary events (a photon crawling along the
+            // shared edge of two faces advances by the rounding-level t of each
+            // crossing, some 10^4-10^5 of them) is served at once rather than
+            // waiting for a full batch: only how soon, never what, changes
+            const bool urgent = __any(search && ((uint32_t)ph.fetal >> 1) >= kUrgentRun);
             const bool at_cap = it + 1 >= a.free_iters;
-            const bool do_list = nlist > 0 && (stuck || at_cap || nlist >= a.list_min * nlive);
-            const bool do_bvh = nbvh > 0 && (stuck || at_cap || nbvh >= a.search_min * nlive);
+            const bool do_list = nlist > 0 && (stuck || at_cap || urgent || nlist >= a.list_min * nlive);
+            const bool do_bvh = nbvh > 0 && (stuck || at_cap || urgent || nbvh >= a.search_min * nlive);
             ++r_iter;
             if (do_list) {
                 ++r_lr;

@@ -676,6 +676,9 @@ __device__ __forceinline__ void jac_dda(const KArgs& a, unsigned long long* jd, 
 // boundary events in a row after which a lane's searches no longer wait
 // for a full batch
 constexpr uint32_t kUrgentRun = 16;
+// live lanes at or below which a wavefront that has no photons left to start
+// runs its searches at once (its last photons are the launch's tail)
+constexpr int kTailLanes = 16;
 
 // per-wavefront event counters in LDS
 enum WaveCount {
@@ -1341,7 +1344,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             // shared edge of two faces advances by the rounding-level t of each
             // crossing, some 10^4-10^5 of them) is served at once rather than
             // waiting for a full batch: only how soon, never what, changes
-            const bool urgent = __any(search && ((uint32_t)ph.fetal >> 1) >= kUrgentRun);
+            const bool urgent = __any(search && ((uint32_t)ph.fetal >> 1) >= kUrgentRun) ||
+                                (exhausted && nlive <= kTailLanes);
             const bool at_cap = it + 1 >= a.free_iters;
             const bool do_list = nlist > 0 && (stuck || at_cap || urgent || nlist >= a.list_min * nlive);
             const bool do_bvh = nbvh > 0 && (stuck || at_cap || urgent || nbvh >= a.search_min * nlive);

@@ -63,7 +63,7 @@ struct lp_context {
     int32_t blocks_per_cu = 0;  // 0: as many as one CU holds at once (the variant's occupancy)
     int32_t free_iters = 32;
     float search_min = 1.0f;
-    float list_min = 0.6f;
+    float list_min = 0.7f;
     int32_t refill_min = 16;
     float clear_step = 0.0f;
     int32_t list_long = 1;
@@ -1040,7 +1040,7 @@ int lp_autotune(lp_context* c, int64_t sample, int32_t passes, lp_tune_result* o
     std::vector<Knob> knobs = {{"waves", {1, 4, 5, 6}, &w},
                                {"free_iters", {16, 32, 64}, &fi},
                                {"refill_min", {8, 16, 24, 32}, &rm},
-                               {"list_min", {0.25, 0.4, 0.5, 0.6, 0.7}, &lm},
+                               {"list_min", {0.4, 0.5, 0.6, 0.7, 0.75}, &lm},
                                {"search_min", {0.5, 0.75, 1.0}, &sm}};
     auto apply = [&]() {
         c->waves = (int32_t)w;

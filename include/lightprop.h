@@ -308,6 +308,11 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
                                    marches on along its ray through the near-face grid
                                    -- cell clearances over empty space, list queries
                                    near faces -- (1, default) or searches the BVH (0) */
+#define LP_OPT_SECOND_STEP 19   /* extra steps per wavefront iteration (0..8, default
+                                   3): a lane whose step was an interaction draws its
+                                   next step at once and takes it there when it stays
+                                   in its free region, up to this many times; else the
+                                   drawn step waits for the next iteration           */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
 
 /* ---- batched launch (replaces ExecKernel's loop of 6 400-photon launches,

@@ -69,6 +69,7 @@ struct lp_context {
     int32_t list_long = 1;
     int32_t no_skip = 0;
     int32_t march = 1;
+    int32_t second_step = 3;
     int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
     float jac_origin[3] = {0, 0, 0}, jac_size[3] = {1, 1, 1};
@@ -340,6 +341,7 @@ void fill_args(lp_context* c, KArgs& a)
     a.list_long = c->list_long;
     a.no_skip = c->no_skip;
     a.march = c->march;
+    a.second_step = c->second_step;
     a.refill_min = c->refill_min;
     a.waves = c->waves;
     a.safe_margin = 2e-6f * c->bvh_extent;
@@ -717,6 +719,10 @@ int lp_set_option(lp_context* c, int32_t key, double v)
     case LP_OPT_MARCH:
         if (v != 0 && v != 1) return fail(c, LP_EINVAL, "march is 0 or 1");
         c->march = (int32_t)v;
+        return LP_OK;
+    case LP_OPT_SECOND_STEP:
+        if (!(v >= 0 && v <= 8 && v == (int32_t)v)) return fail(c, LP_EINVAL, "second_step is 0..8");
+        c->second_step = (int32_t)v;
         return LP_OK;
     case LP_OPT_NO_SKIP:
         if (v != 0 && v != 1) return fail(c, LP_EINVAL, "no_skip is 0 or 1");

@@ -157,6 +157,7 @@ struct KArgs {
     int32_t no_skip;       // 1: search every step (no free region, clearance, safety grid or distance query)
     int32_t march;         // 1: a long step the lists miss marches on through the grid (cell clearances and
                            // further list queries) instead of searching the BVH
+    int32_t second_step;   // extra steps a lane that took an interaction may take in the same iteration
     int32_t refill_min;    // end the round once this many lanes of a wavefront have finished
     int32_t waves;         // occupancy target of the kernel variant (1, 4, 5, 6)
     float nmuasg[4 * kMaxTissues];

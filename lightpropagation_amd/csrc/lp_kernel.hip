@@ -1533,6 +1533,54 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             r_int += (uint32_t)__popcll(__ballot(go && !bnd));
             r_bnd += (uint32_t)__popcll(__ballot(go && bnd));
             r_skip += (uint32_t)__popcll(__ballot(skipped));
+            // ---- further steps in the same iteration ----
+            // A lane whose step was an interaction draws its next step now and
+            // takes it at once when its end point stays in the free region (the
+            // same test as at the top of the iteration; the cell clearance is
+            // not tried here), up to second_step times.  Otherwise the drawn
+            // step waits for the next iteration, which finds have_step set.
+            // The per-iteration work of the wavefront (refill, random top-up,
+            // batch decisions, its scalar bookkeeping) is then shared by several
+            // steps of most lanes; each photon's steps, draws and deposits are
+            // the same as with one step per iteration.
+            bool prev = go && !bnd;  // this lane's last step was an interaction
+            // (not while a lane of the wavefront crawls along an edge: each of
+            // its crossings takes an iteration, so iterations stay short)
+            const int extra = __any(alive && ((uint32_t)ph.fetal >> 1) >= kUrgentRun) ? 0 : a.second_step;
+            for (int k2 = 0; k2 < extra && __any(prev && alive && !ballistic); ++k2) {
+                const bool cand = prev && alive && !ballistic;
+                bool go2 = false;
+                if (cand) rng.top_up();
+                if (cand) {
+                    const float4 cm = lmat[ph.inst * ntp1 + ph.curr];
+                    if (ph.unfin == 0.0f) {
+                        float x;
+                        do {
+                            x = rng.next();
+                        } while (x <= 0.0f);
+                        stepsize = -logf_u01(x) * a.factor / (cm.y + cm.z);
+                    } else {
+                        stepsize = ph.unfin / (cm.y + cm.z);
+                        ph.unfin = 0.0f;
+                    }
+                    have_step = true;
+                    const f3 q = add(ph.p, mul(stepsize, ph.v));
+                    const f3 e = sub(q, ph.anc);
+                    go2 = !a.no_skip && dot(e, e) < ph.safe * ph.safe && dot(ph.pu, q) > ph.pc;
+                }
+                if (go2) {
+                    interaction(stepsize);
+                    have_step = false;
+                    if (!end_step(0, -1, stepsize)) {
+                        epilogue();
+                        alive = false;
+                    }
+                }
+                const uint32_t n2 = (uint32_t)__popcll(__ballot(go2));
+                r_int += n2;
+                r_skip += n2;
+                prev = go2;
+            }
             // end the round at the cap (finished lanes are refilled then), or
             // once enough lanes have finished and photons are left to start
             if (at_cap || (!exhausted && 64 - __popcll(__ballot(alive)) >= a.refill_min)) break;

@@ -110,12 +110,25 @@ def cpu_baseline(sc, seconds=20.0):
             max_mhz = int(f.read().strip()) / 1000.0
     except (OSError, ValueError):
         pass
+    lscpu = {}
+    try:  # lscpu's view of the host: model, clock, sockets and cores
+        import subprocess
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=10).stdout
+        want = ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)", "CPU max MHz",
+                "CPU MHz", "BogoMIPS", "L3 cache", "Hypervisor vendor")
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() in want:
+                lscpu[k.strip()] = v.strip()
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
     return {"value": done / dt, "unit": "photons/s", "cores": cores, "kind": "port",
             "sample": "%d photons of the same workload (660 nm; %d steps) in %.1f s, one photon per OpenMP "
                       "thread; photon-steps/s %.3g" % (done, steps, dt, steps / dt),
             "host": {"cpu_model": model, "cpu_mhz_now_mean": (sum(mhz) / len(mhz)) if mhz else None,
                      "cpu_mhz_max": max_mhz, "cores_available": avail, "logical_cpus": os.cpu_count(),
-                     "threads_used": cores}}
+                     "threads_used": cores, "lscpu": lscpu}}
 
 
 def adjoint_vs_fd(sc, photons=10_000_000, batches=16, rels=(1e-3, 2e-2), jac_n=64):

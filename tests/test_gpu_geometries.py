@@ -138,6 +138,28 @@ def test_batched_geometries_against_oracle():
 C2_GEOMS = [(1.5, 19, 14, {"t_fat": 0.3, "t_uterus": 0.5}), (2.5, 38, 27, {}), (3.5, 45, 32, {}), (4.5, 53, 38, {})]
 
 
+def test_autotune_on_merged_geometries_keeps_results():
+    # the autotuner's trial launches on a context holding several
+    # geometries (transport in geometry 0) leave its tallies untouched, and
+    # the batched launch over the geometries gives the same tallies with the
+    # options it picks as with the defaults
+    scs = _scenes()
+    merged, counts = S.geometries(scs)
+    tab = _table()
+    geom = (np.arange(len(tab)) % len(counts)).astype(np.int32)
+    ctx = _ctx(merged, 0x5EED0031, counts)
+    before = ctx.run_batch(tab, 0, 30_000, geom=geom)
+    r = ctx.autotune(sample=300_000, passes=1)
+    print("autotune: %d trials, %.1f -> %.1f ms; waves %d, free %d, refill %d, list %.3g"
+          % (r.trials, r.ms_before, r.ms_after, r.waves, r.free_iters, r.refill_min, r.list_min), flush=True)
+    assert r.trials >= 2 and r.ms_after <= r.ms_before
+    after = ctx.run_batch(tab, 0, 30_000, geom=geom)
+    ctx.close()
+    for k in ("det", "region"):
+        assert np.array_equal(before[k], after[k]), k
+    assert before["det"][:, :, 0].sum() > 0
+
+
 def test_batched_workload_at_full_size():
     # the batched workload's real shape -- 4 merged geometries (56 480 to
     # 377 120 faces), 2 wavelengths x 16 fetal SpO2 per geometry = 128

@@ -54,8 +54,8 @@ extern "C" {
                              tests, skipped searches, NaN photons, free-radius
                              queries, their node visits, list face tests,
                              wavefront iterations, near-list search batches,
-                             BVH search batches, replay mismatches, marches
-                             (32 slots)                                       */
+                             BVH search batches, replay mismatches, marches,
+                             edge-crawl crossings replayed (32 slots)         */
 #define LP_NTALLY 9
 
 /* Tally values are int64 fixed point: 2^32 per unit photon weight, except
@@ -313,6 +313,12 @@ int lp_set_launch(lp_context *ctx, int32_t grid_blocks, int32_t lds_nodes, int32
                                    next step at once and takes it there when it stays
                                    in its free region, up to this many times; else the
                                    drawn step waits for the next iteration           */
+#define LP_OPT_CRAWL 20         /* replay edge crawls (1, default): a photon on the
+                                   shared edge of two faces, crossing them in turn
+                                   without moving, takes the repeats of its last few
+                                   crossings from per-lane records within one
+                                   iteration (same draws, states and tallies) instead
+                                   of one crossing per wavefront iteration; 0: off  */
 int lp_set_option(lp_context *ctx, int32_t key, double value);
 
 /* ---- batched launch (replaces ExecKernel's loop of 6 400-photon launches,

@@ -16,11 +16,11 @@ RNG_PHILOX, RNG_RAN3 = 0, 1
 TALLY = {"sft": 0, "rtd": 1, "arz": 2, "phirz": 3, "region": 4, "vox": 5, "rad": 6, "det": 7, "misc": 8}
 MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped", "arz_out", "vox_out",
         "node_visits", "tri_tests", "skipped", "nan", "dist_queries", "dist_visits", "list_tests", "wave_iters", "search_rounds",
-        "bvh_rounds", "replay_mismatch", "marches"]
+        "bvh_rounds", "replay_mismatch", "marches", "crawl_replayed"]
 OPTIONS = {"grid_blocks": 0, "lds_nodes": 1, "safety_cells": 2, "dist_cap": 3, "dist_cap_steps": 4,
            "blocks_per_cu": 5, "free_iters": 6, "waves_per_simd": 7, "near_r0": 8, "near_cell": 9, "search_min": 10,
            "list_min": 11, "refill_min": 12, "graph": 13, "clear_step": 14, "boundary_run": 15, "list_long": 16,
-           "no_skip": 17, "march": 18, "second_step": 19}
+           "no_skip": 17, "march": 18, "second_step": 19, "crawl": 20}
 FIXED = 2.0 ** 32
 FIXED_PHI = 2.0 ** 24
 

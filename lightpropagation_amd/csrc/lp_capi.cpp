@@ -70,6 +70,7 @@ struct lp_context {
     int32_t no_skip = 0;
     int32_t march = 1;
     int32_t second_step = 3;
+    int32_t crawl = 1;  // replay edge crawls from the lanes' crossing records
     int32_t waves = 4;
     float bvh_extent = 0.0f;  // largest |coordinate| of the mesh
     float jac_origin[3] = {0, 0, 0}, jac_size[3] = {1, 1, 1};
@@ -88,6 +89,8 @@ struct lp_context {
 
     // device
     DevBuf d_tris, d_nodes, d_nrm, d_med, d_safety, d_vstart, d_vocc, d_counter, d_exit;
+    DevBuf d_crawl;  // per-lane records of edge-crawl crossings (kCrawlSlots x kCrawlWords words per lane)
+    int64_t crawl_lanes = 0;
     DevBuf d_tally[LP_NTALLY];
     DevBuf d_tune[LP_NTALLY];  // the autotuner's stand-ins for the tallies (same sizes)
     DevBuf d_det_list, d_det_count, d_grad, d_jac, d_jac_coll, d_adj_misc;
@@ -306,6 +309,11 @@ int prepare(lp_context* c)
         c->tallies_dirty = false;
     }
     if ((rc = dev_alloc(c, c->d_counter, 64))) return rc;
+    if (!c->d_crawl.p) {  // room for every lane of the largest automatic grid (8 work-groups per CU)
+        const int64_t lanes = (int64_t)std::max(c->num_cus, 1) * 8 * kBlock;
+        if ((rc = dev_alloc(c, c->d_crawl, (size_t)lanes * kCrawlSlots * kCrawlWords * sizeof(float)))) return rc;
+        c->crawl_lanes = lanes;
+    }
     return LP_OK;
 }
 
@@ -314,6 +322,8 @@ void fill_args(lp_context* c, KArgs& a)
     std::memset(&a, 0, sizeof a);
     a.tris = (const TriRec*)c->d_tris.p;
     a.nodes = (const BvhNode*)c->d_nodes.p;
+    a.crawl = c->crawl ? (float*)c->d_crawl.p : nullptr;
+    a.crawl_lanes = c->crawl_lanes;
     a.nnodes = (int32_t)c->bvh.nodes.size();
     // a launch over several geometries stages fewer BVH nodes, leaving LDS
     // for the larger material tables of its many instances
@@ -490,7 +500,7 @@ void lp_destroy(lp_context* c)
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
                       &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list,
                       &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_jac_coll, &c->d_adj_misc, &c->d_inst_mat,
-                      &c->d_bdet, &c->d_bregion, &c->d_pair_len, &c->d_pair_int};
+                      &c->d_bdet, &c->d_bregion, &c->d_pair_len, &c->d_pair_int, &c->d_crawl};
     for (DevBuf* b : bufs) dev_free(*b);
     for (auto& b : c->d_tally) dev_free(b);
     for (auto& b : c->d_tune) dev_free(b);
@@ -724,6 +734,10 @@ int lp_set_option(lp_context* c, int32_t key, double v)
         if (!(v >= 0 && v <= 8 && v == (int32_t)v)) return fail(c, LP_EINVAL, "second_step is 0..8");
         c->second_step = (int32_t)v;
         return LP_OK;
+    case LP_OPT_CRAWL:
+        if (!(v == 0 || v == 1)) return fail(c, LP_EINVAL, "crawl is 0 or 1");
+        c->crawl = (int32_t)v;
+        return LP_OK;
     case LP_OPT_NO_SKIP:
         if (v != 0 && v != 1) return fail(c, LP_EINVAL, "no_skip is 0 or 1");
         c->no_skip = (int32_t)v;
@@ -954,6 +968,7 @@ int lp_run_batch_noalloc(lp_context* c, const float* nmuasg, int32_t ninst, cons
     fill_args(c, a);
     a.sft = a.rtd = a.arz = a.phirz = a.vox = a.rad = nullptr;
     a.misc = nullptr;  // no event counters: the context's tallies are not touched
+    a.crawl = nullptr;  // nor the context's crawl records (the crawl fast path is off)
     a.det = nd > 0 ? (unsigned long long*)det_dev : nullptr;
     a.region = (unsigned long long*)region_dev;
     a.counter = (unsigned long long*)sp;
@@ -1411,7 +1426,7 @@ int lp_get_stats(lp_context* c, lp_stats* out)
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
                       &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list,
                       &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_jac_coll, &c->d_adj_misc, &c->d_inst_mat,
-                      &c->d_bdet, &c->d_bregion, &c->d_pair_len, &c->d_pair_int};
+                      &c->d_bdet, &c->d_bregion, &c->d_pair_len, &c->d_pair_int, &c->d_crawl};
     for (DevBuf* b : bufs) bytes += (int64_t)b->bytes;
     for (auto& b : c->d_tally) bytes += (int64_t)b.bytes;
     c->stats.device_bytes = bytes;

@@ -224,6 +224,10 @@ struct KArgs {
     // work
     int64_t first, count;
     unsigned long long* counter;
+    // per-lane records of the last two crossings of an edge crawl (null: the
+    // crawl fast path is off); lanes at or past crawl_lanes do without
+    float* crawl;
+    int64_t crawl_lanes;
 };
 
 // A detected photon: absolute photon index, detector, R exit weight.
@@ -235,12 +239,20 @@ struct DetRec {
 static_assert(sizeof(DetRec) == 16, "detection record layout");
 
 constexpr int kAdjMaxTissues = 16;  // tissues the adjoint replay tracks per lane (LDS)
+// A crossing record of an edge crawl, 32-bit words: photon index (lo, hi),
+// step number, the input direction (3), last face and tissue, then the
+// outcome: face, t, Fresnel coefficient, output direction (3), tissue, normal
+// code, kind (1 transmit, 2 reflect), and mua + mus of the input tissue.
+// Each lane keeps the records of its last kCrawlSlots steps (a ring indexed
+// by step number).
+constexpr int kCrawlWords = 18;
+constexpr int kCrawlSlots = 4;  // a power of 2
 
 enum MiscSlot {  // slots of the misc tally (kMiscN int64 words)
     kMiscPhotons = 0, kMiscSteps, kMiscInteract, kMiscBoundary, kMiscEscaped, kMiscCapped,
     kMiscArzOut, kMiscVoxOut, kMiscNodeVisits, kMiscTriTests, kMiscSkipped, kMiscNan, kMiscDistQueries,
     kMiscDistVisits, kMiscListTests, kMiscIters, kMiscRounds, kMiscBvhRounds, kMiscReplayMismatch, kMiscMarches,
-    kMiscUsed,
+    kMiscCrawl, kMiscUsed,
     kMiscN = 32
 };
 

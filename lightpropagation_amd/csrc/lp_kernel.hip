@@ -117,6 +117,8 @@ struct PhiloxState {
         x0 = (float)(u0 >> 8) * 5.96046448e-08f;
         x1 = (float)(u1 >> 8) * 5.96046448e-08f;
     }
+    // the value next() returns next, not consumed (after top_up: left >= 4)
+    __device__ __forceinline__ float peek() const { return (float)(b0 >> 8) * 5.96046448e-08f; }
 };
 
 // PSNR::InitRand3 (src/PSNR/PSNR.cpp:48-80) in int32 (all values < 2^31),
@@ -190,6 +192,17 @@ struct Rng {
         if (COUNT) ndraw++;
         if (RNG == 0) return ph.next(k0, k1);
         return ran3f_next(s, tid, inext, inextp);
+    }
+    // the value next() returns next, not consumed (Philox: after top_up)
+    __device__ __forceinline__ float peek() const
+    {
+        if (RNG == 0) return ph.peek();
+        int i = inext + 1, j = inextp + 1;
+        if (i == 56) i = 1;
+        if (j == 56) j = 1;
+        float r = __int_as_float(s[i * kBlock + tid]) - __int_as_float(s[j * kBlock + tid]);
+        if (r < 0.0f) r = r + 1000000000.0f;
+        return r / 1000000000.0f;
     }
     __device__ __forceinline__ void next2(float& x0, float& x1)
     {
@@ -683,7 +696,7 @@ constexpr int kTailLanes = 16;
 // per-wavefront event counters in LDS
 enum WaveCount {
     wPhot, wInt, wBnd, wEsc, wCap, wSkip, wDq, wNan, wLst, wIter, wLRounds, wBRounds, wArzOut, wVoxOut, wMismatch,
-    wMarch, wVis, wTst, wDvis, kWc
+    wMarch, wCrawl, wVis, wTst, wDvis, kWc
 };
 
 // WAVES: minimum waves per SIMD the register allocator must allow (1 =
@@ -758,6 +771,19 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
     rng.tid = tid;
     rng.k0 = (uint32_t)a.seed;
     rng.k1 = (uint32_t)(a.seed >> 32);
+    // each lane's two records of edge-crawl crossings: those of an earlier
+    // launch (other optics, other photons) are voided before any is read
+    if constexpr (!ADJ) {
+        const uint32_t gid = blockIdx.x * kBlock + tid;
+        if (a.crawl && (int64_t)gid < a.crawl_lanes) {
+            uint32_t* u = reinterpret_cast<uint32_t*>(a.crawl + (size_t)gid * kCrawlSlots * kCrawlWords);
+#pragma unroll
+            for (int i = 0; i < kCrawlSlots; ++i) {
+                u[i * kCrawlWords] = 0xffffffffu;
+                u[i * kCrawlWords + 1] = 0xffffffffu;
+            }
+        }
+    }
     Photon ph;
     bool alive = false;
     bool exhausted = false;  // wave-uniform
@@ -792,6 +818,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
     bool have_step = false;  // a step length has been drawn and not yet used
     bool search = false;     // the pending step needs a face search
     bool by_list = false;    // ... from the near-face lists (else from the BVH)
+    bool crawled = false;    // this iteration's boundary event left p where it was (and was recorded)
 
     // weights recorded at the exit (R, fetal R, T): set by the boundary event
     // that ends the photon, read by the epilogue in the same iteration
@@ -927,6 +954,28 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
         const f3 p_old = ph.p;
         ph.p = add(ph.p, mul(stb, ph.v));  // MoveStep
         if constexpr (ADJ) adj_step(p_old, stb, false, 0.0f);
+        // A move below the float spacing of p leaves p unchanged (a photon on
+        // the shared edge of two faces): such a crossing is recorded, input
+        // state first, in the lane's ring slot for its step number.
+        float* crec = nullptr;
+        if constexpr (!ADJ) {
+            const uint32_t gid = blockIdx.x * kBlock + tid;
+            if (a.crawl && h.id >= 0 && (int64_t)gid < a.crawl_lanes &&
+                __float_as_uint(ph.p.x) == __float_as_uint(p_old.x) &&
+                __float_as_uint(ph.p.y) == __float_as_uint(p_old.y) &&
+                __float_as_uint(ph.p.z) == __float_as_uint(p_old.z)) {
+                crec = a.crawl + ((size_t)gid * kCrawlSlots + (ph.steps & (kCrawlSlots - 1))) * kCrawlWords;
+                uint32_t* u = reinterpret_cast<uint32_t*>(crec);
+                u[0] = (uint32_t)ph.k;
+                u[1] = (uint32_t)((uint64_t)ph.k >> 32);
+                u[2] = ph.steps;
+                crec[3] = ph.v.x;
+                crec[4] = ph.v.y;
+                crec[5] = ph.v.z;
+                u[6] = (uint32_t)ph.ref;
+                u[7] = (uint32_t)ph.curr;
+            }
+        }
         ph.ref = -1;
         ph.safe = 0.0f;
         if (h.id < 0) {  // nothing ahead: the photon has left the mesh
@@ -1014,13 +1063,31 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             ph.ref = h.id;
             if (ph.curr == a.fetal_id) ph.fetal |= 1;
             ph.unfin = (s - stb) * (cm.y + cm.z);
+            if constexpr (!ADJ) {
+                if (crec) {
+                    uint32_t* u = reinterpret_cast<uint32_t*>(crec);
+                    u[8] = (uint32_t)h.id;
+                    crec[9] = stb;
+                    crec[10] = r;
+                    crec[11] = ph.v.x;
+                    crec[12] = ph.v.y;
+                    crec[13] = ph.v.z;
+                    u[14] = (uint32_t)ph.curr;
+                    u[15] = (uint32_t)ph.nvc;
+                    u[16] = (uint32_t)kind;
+                    crec[17] = cm.y + cm.z;
+                    crawled = true;
+                }
+            }
         }
         return kind;
     };
 
     // Roulette (.cl:1020-1035, 1539-1545), trace, step cap; false when the
     // photon has ended
-    auto end_step = [&](int kind, int hid, float s) -> bool {
+    // (inlined at each of its call sites: an out-of-line call would take the
+    // photon state by address, into scratch memory)
+    auto end_step = [&](int kind, int hid, float s) __attribute__((always_inline)) -> bool {
         if (ph.w < a.w_th && ph.w > 0.0f) {
             float x = rng.next();
             if (x < a.chance) {
@@ -1058,6 +1125,82 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             ph.w = 0.0f;
         }
         return ph.w > 0.0f && (ph.w > a.w_th || ph.roulette);
+    };
+
+    // An edge crawl, fast-forwarded.  A photon on the shared edge of two
+    // faces crosses them in turn without moving, its state (direction, last
+    // face, tissue) running through a short cycle.  When the crossings of
+    // steps n - L .. n - 1 (L <= kCrawlSlots) are all recorded and the first
+    // one's input state is the present one, the crossings that follow repeat
+    // them in order: the nearest face ahead, its t, the Fresnel coefficient,
+    // the outgoing direction, tissue and normal are functions of that state
+    // and of the launch's optics alone.  The step length is recomputed from
+    // the unfinished optical depth as the step draw computes it, and each
+    // crossing consumes its Fresnel draw from the stream as the boundary event
+    // does.  The replay stops, leaving the next crossing to the general path,
+    // when the step no longer reaches the face, when the draw picks the other
+    // outcome, when the weight lies below the roulette threshold (roulette
+    // would draw), or when no optical depth is left; end_step ends the photon
+    // as it would (step or run cap).  Draws, states, tallies and trace rows are
+    // those of taking the crossings one iteration at a time.  False when the
+    // photon has ended.
+    auto crawl_forward = [&]() -> bool {
+        const uint32_t gid = blockIdx.x * kBlock + tid;
+        const float* ring = a.crawl + (size_t)gid * kCrawlSlots * kCrawlWords;
+        const uint32_t k0 = (uint32_t)ph.k, k1 = (uint32_t)((uint64_t)ph.k >> 32);
+        // the shortest cycle: a record of this photon (in this launch: the
+        // kernel voids older ones) for each of the last L steps, the first of
+        // them leaving from the present state
+        uint32_t L = 0;
+#pragma unroll
+        for (uint32_t l = 2; l <= (uint32_t)kCrawlSlots; ++l) {
+            const uint32_t* u =
+                reinterpret_cast<const uint32_t*>(ring + ((ph.steps - l) & (kCrawlSlots - 1)) * kCrawlWords);
+            if (u[0] != k0 || u[1] != k1 || u[2] != ph.steps - l) break;  // not recorded: no cycle
+            if (u[3] == __float_as_uint(ph.v.x) && u[4] == __float_as_uint(ph.v.y) &&
+                u[5] == __float_as_uint(ph.v.z) && (int32_t)u[6] == ph.ref && (int32_t)u[7] == ph.curr) {
+                L = l;
+                break;
+            }
+        }
+        if (L == 0) return true;
+        const uint32_t first = ph.steps - L;  // the step whose record is replayed first
+        uint32_t j = 0;                       // ... and the place in the cycle of the next crossing
+        uint32_t nff = 0;
+        bool live = true;
+        for (;;) {
+            const float* rx = ring + ((first + j) & (kCrawlSlots - 1)) * kCrawlWords;
+            const uint32_t* ux = reinterpret_cast<const uint32_t*>(rx);
+            const float t = rx[9], r = rx[10], m = rx[17];
+            if (ph.unfin == 0.0f || !(m > 0.0f) || ph.w < a.w_th) break;
+            const float s = ph.unfin / m;  // the step draw's division
+            if (!(t <= s)) break;           // the face lies beyond the step: an interaction
+            const int32_t kd = (int32_t)ux[16];
+            rng.top_up();
+            if ((rng.peek() > r) != (kd == 1)) break;  // the other outcome
+            (void)rng.next();
+            const int32_t c = (int32_t)ux[14];
+            if (c != ph.curr) {
+                if (a.region && reg_acc) atomic_add_u64(&a.region[ph.inst * a.ntissue + ph.curr], reg_acc);
+                reg_acc = 0;
+            }
+            ph.v = mk(rx[11], rx[12], rx[13]);
+            ph.nvc = (int32_t)ux[15];
+            ph.curr = c;
+            const int32_t f = (int32_t)ux[8];
+            ph.ref = f;
+            if (ph.curr == a.fetal_id) ph.fetal |= 1;
+            ph.unfin = (s - t) * m;
+            ++nff;
+            if (!end_step(kd, f, s)) {
+                live = false;
+                break;
+            }
+            j = (j + 1 == L) ? 0u : j + 1;
+        }
+        wsum(wBnd, nff);
+        wsum(wCrawl, nff);
+        return live;
     };
 
     // epilogue (.cl:1586-1594) + host binning (cpp:1553-1630)
@@ -1260,6 +1403,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             bool freed = false;  // ... and the step was shown to stay in the free region
             bool skipped = false;  // ... with no face search
             bool bnd = false;
+            crawled = false;
             Hit h;
             h.id = -1;
             h.t = 10000.0f;
@@ -1525,7 +1669,11 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
                     if (!freed && !(dot(e, e) < ph.safe * ph.safe && dot(ph.pu, ph.p) > ph.pc)) ph.safe = 0.0f;
                 }
                 have_step = false;
-                if (!end_step(kind, bnd ? h.id : -1, stepsize)) {
+                bool live = end_step(kind, bnd ? h.id : -1, stepsize);
+                if constexpr (!ADJ) {
+                    if (crawled && live && ((uint32_t)ph.fetal >> 1) >= 2) live = crawl_forward();
+                }
+                if (!live) {
                     epilogue();
                     alive = false;
                 }
@@ -1615,6 +1763,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void transport_kernel(KArgs a_arg)
             atomic_add_u64(&a.misc[kMiscVoxOut], wc[wVoxOut]);
             atomic_add_u64(&a.misc[kMiscReplayMismatch], wc[wMismatch]);
             atomic_add_u64(&a.misc[kMiscMarches], wc[wMarch]);
+            atomic_add_u64(&a.misc[kMiscCrawl], wc[wCrawl]);
             atomic_add_u64(&a.misc[kMiscNodeVisits], wc[wVis]);
             atomic_add_u64(&a.misc[kMiscTriTests], wc[wTst]);
             atomic_add_u64(&a.misc[kMiscDistVisits], wc[wDvis]);

@@ -252,7 +252,7 @@ PHYSICS_MISC = ["photons", "steps", "interact", "boundary", "escaped", "capped",
 def test_tuning_options_do_not_change_results():
     # Every option of lp_set_option only changes how the kernel finds faces
     # (free regions, near-face lists, BVH in LDS, search scheduling, register
-    # budget, steps per iteration), never which face a step meets: 10^6
+    # budget, steps per iteration, crawl replay), never which face a step meets: 10^6
     # photons of the bench workload (196 420 faces) give bit-identical
     # tallies under each setting.
     from lightpropagation_amd import scenes as S
@@ -262,7 +262,7 @@ def test_tuning_options_do_not_change_results():
                 {"dist_cap": 0.0}, {"clear_step": -1.0, "refill_min": 4}, {"clear_step": 0.05},
                 {"list_long": 0}, {"list_long": 0, "near_r0": 0.03, "dist_cap": 0.0}, {"no_skip": 1},
                 {"no_skip": 1, "near_r0": 0.0}, {"second_step": 0}, {"second_step": 1, "list_min": 0.3},
-                {"second_step": 8, "waves_per_simd": 5}]
+                {"second_step": 8, "waves_per_simd": 5}, {"crawl": 0}]
     ref = None
     for opts in settings:
         ctx = lp.Context(0)

@@ -504,7 +504,8 @@ def main():
         steps += m["steps"]
         flops += m["interact"] * (F_INT + search) + m["boundary"] * (F_BND + search)
         for k in ("interact", "boundary", "escaped", "capped", "nan", "node_visits", "tri_tests", "skipped",
-                  "dist_queries", "dist_visits", "list_tests", "wave_iters", "search_rounds", "bvh_rounds"):
+                  "dist_queries", "dist_visits", "list_tests", "wave_iters", "search_rounds", "bvh_rounds",
+                  "marches", "crawl_replayed"):
             counters[k] = counters.get(k, 0) + m[k]
         if n:
             R = t["sft"].reshape(-1, 3).sum(0) / lp.FIXED / n

@@ -85,7 +85,7 @@ def cpu_baseline(sc, seconds=20.0):
     except AttributeError:
         cores = os.cpu_count() or 1
     avail = cores
-    cores = max(1, min(cores, 16))  # the GPU box's CPU share is 16 cores, whatever the host has
+    cores = max(1, min(cores, 16))  # at most 16 threads, whatever the host has
     run = O.Run(sc, seed=2024)
     done, steps, t0 = 0, 0, time.perf_counter()
     batch = cores

@@ -7,7 +7,7 @@ lane's records) and off; and a batched launch that holds it costs no more
 than one photon's 10 000 extra steps.  On the bench workload (196 420
 faces, 660 nm), photons crawling along an edge in cycles of two and of four
 crossings give the same step traces, exit records and tallies with the
-replay on and off."""
+replay on and off, with the Philox streams and with ran3."""
 import os
 import sys
 
@@ -105,6 +105,36 @@ def test_headline_range_with_crawlers_same_tallies():
         for k in ("photons", "steps", "interact", "boundary", "escaped", "capped", "nan"):
             assert m1[k] == m0[k], k
         assert m1["capped"] >= 1 and m1["crawl_replayed"] > 90_000 and m0["crawl_replayed"] == 0
+
+
+# under the ran3 streams of the same workload, photons [93 650 000,
+# 93 700 000) hold a crawler that reaches the cap: the replay peeks each
+# Fresnel draw from the lane's ran3 state before consuming it
+RAN3_RANGE = (93_650_000, 50_000)
+
+
+def test_ran3_range_with_crawler_same_tallies():
+    out = {}
+    for crawl in (1, 0):
+        ctx = _bench_ctx(crawl)
+        from lightpropagation_amd import scenes as S
+        sc = S.maternal_fetal(wavelength_nm=660.0)
+        ctx.set_params(float(sc["chance"]), float(sc["w_th"]), 1.0, 0x5EED0001, lp.RNG_RAN3, 0)
+        ctx.set_tallies(sum(1 << k for k in (0, 1, 4, 5, 6, 7, 8)), record_exit=True)
+        print("ran3 range", RAN3_RANGE, "crawl", crawl, flush=True)
+        ctx.reset()
+        ctx.run(*RAN3_RANGE)
+        t = ctx.tallies()
+        out[crawl] = ({k: np.asarray(v).copy() for k, v in t.items() if k != "misc"}, dict(t["misc"]),
+                      ctx.exit_records(RAN3_RANGE[1]).copy())
+        ctx.close()
+    (t1, m1, e1), (t0, m0, e0) = out[1], out[0]
+    for k in t1:
+        assert np.array_equal(t1[k], t0[k]), k
+    assert np.array_equal(e1.view(np.uint32), e0.view(np.uint32))
+    for k in ("photons", "steps", "interact", "boundary", "escaped", "capped", "nan"):
+        assert m1[k] == m0[k], k
+    assert m1["capped"] == 1 and m1["crawl_replayed"] > 90_000 and m0["crawl_replayed"] == 0
 
 
 def test_batch_holding_the_trapped_photon():

@@ -38,6 +38,8 @@ def ctx():
 def _scene(name):
     if name == "c0slab":
         return scenes.slab(scenes.C0_LAYERS)
+    if name == "c0slab_g1":  # g = 1 - 2^-24 in the middle layer (forward scattering at the float limit)
+        return scenes.slab([L if i != 1 else L[:4] + (0.99999994,) for i, L in enumerate(scenes.C0_LAYERS)])
     if name == "mf_coarse":  # maternal-fetal abdomen, 5 040 faces
         from lightpropagation_amd import scenes as S
         return S.maternal_fetal(freq_maternal=6, freq_fetal=6, freq_torso=4)
@@ -122,7 +124,7 @@ def test_step_log_equals_general_log(ctx):
 
 
 CASES = [("model1", 512), ("model2", 512), ("model6FB", 256), ("c0slab", 256), ("model4", 48), ("model3", 256),
-         ("planar", 128), ("escape", 256), ("oblique", 256)]
+         ("planar", 128), ("escape", 256), ("oblique", 256), ("c0slab_g1", 256)]
 
 
 @pytest.mark.parametrize("rng", [lp.RNG_PHILOX, lp.RNG_RAN3], ids=["philox", "ran3"])
@@ -145,7 +147,7 @@ def test_traces_bit_exact(ctx, name, n, rng):
 
 
 TALLY_CASES = [("model1", 20000), ("model2", 8000), ("c0slab", 3000), ("model4", 96), ("mf_coarse", 1500),
-               ("model3", 4000), ("planar", 1500), ("escape", 3000), ("oblique", 4000)]
+               ("model3", 4000), ("planar", 1500), ("escape", 3000), ("oblique", 4000), ("c0slab_g1", 3000)]
 
 
 @pytest.mark.parametrize("rng", [lp.RNG_PHILOX, lp.RNG_RAN3], ids=["philox", "ran3"])

@@ -968,7 +968,7 @@ int lp_run_batch_noalloc(lp_context* c, const float* nmuasg, int32_t ninst, cons
     fill_args(c, a);
     a.sft = a.rtd = a.arz = a.phirz = a.vox = a.rad = nullptr;
     a.misc = nullptr;  // no event counters: the context's tallies are not touched
-    a.crawl = nullptr;  // nor the context's crawl records (the crawl fast path is off)
+    a.crawl = nullptr;  // nor the context's crawl records (the crawl replay is off)
     a.det = nd > 0 ? (unsigned long long*)det_dev : nullptr;
     a.region = (unsigned long long*)region_dev;
     a.counter = (unsigned long long*)sp;

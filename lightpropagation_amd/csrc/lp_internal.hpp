@@ -224,8 +224,8 @@ struct KArgs {
     // work
     int64_t first, count;
     unsigned long long* counter;
-    // per-lane records of the last two crossings of an edge crawl (null: the
-    // crawl fast path is off); lanes at or past crawl_lanes do without
+    // per-lane rings of the last kCrawlSlots crossings that left p unchanged
+    // (null: the crawl replay is off); lanes at or past crawl_lanes do without
     float* crawl;
     int64_t crawl_lanes;
 };

@@ -138,7 +138,7 @@ int dev_alloc(lp_context* c, DevBuf& b, size_t bytes)
     if (bytes == 0) bytes = 16;
     if (b.p && b.bytes >= bytes) return LP_OK;
     if (b.p) {
-        hipFree(b.p);
+        (void)hipFree(b.p);
         b.p = nullptr;
         b.bytes = 0;
     }
@@ -153,7 +153,7 @@ int dev_alloc(lp_context* c, DevBuf& b, size_t bytes)
 
 void dev_free(DevBuf& b)
 {
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
 }
@@ -493,10 +493,10 @@ int lp_create(lp_context** out, int32_t device)
 void lp_destroy(lp_context* c)
 {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->bexec) hipGraphExecDestroy(c->bexec);
-    if (c->bgraph) hipGraphDestroy(c->bgraph);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->bexec) (void)hipGraphExecDestroy(c->bexec);
+    if (c->bgraph) (void)hipGraphDestroy(c->bgraph);
     DevBuf* bufs[] = {&c->d_tris, &c->d_nodes, &c->d_nrm, &c->d_med, &c->d_safety, &c->d_vstart,
                       &c->d_vocc, &c->d_counter, &c->d_exit, &c->d_near_cells, &c->d_near_tris, &c->d_det_list,
                       &c->d_det_count, &c->d_grad, &c->d_jac, &c->d_jac_coll, &c->d_adj_misc, &c->d_inst_mat,
@@ -504,9 +504,9 @@ void lp_destroy(lp_context* c)
     for (DevBuf* b : bufs) dev_free(*b);
     for (auto& b : c->d_tally) dev_free(b);
     for (auto& b : c->d_tune) dev_free(b);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -836,8 +836,8 @@ int build_inst_table(lp_context* c, const float* nmuasg, int32_t ninst, const in
 int batch_graph(lp_context* c, const KArgs& a, int grid, int64_t ndet_words, int64_t nreg_words)
 {
     if (c->bexec && c->bgrid == grid && std::memcmp(&c->bargs, &a, sizeof a) == 0) return LP_OK;
-    if (c->bexec) hipGraphExecDestroy(c->bexec);
-    if (c->bgraph) hipGraphDestroy(c->bgraph);
+    if (c->bexec) (void)hipGraphExecDestroy(c->bexec);
+    if (c->bgraph) (void)hipGraphDestroy(c->bgraph);
     c->bexec = nullptr;
     c->bgraph = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
@@ -850,13 +850,13 @@ int batch_graph(lp_context* c, const KArgs& a, int grid, int64_t ndet_words, int
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (bad || e != hipSuccess || !g) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return fail(c, LP_EDEVICE, "lp_run_batch: graph capture failed (%s)", err[0] ? err : hipGetErrorString(e));
     }
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         return fail(c, LP_EDEVICE, "lp_run_batch: graph instantiation: %s", hipGetErrorString(e));
     }
     c->bgraph = g;

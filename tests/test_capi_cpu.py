@@ -31,6 +31,17 @@ def test_library_exports(name):
     assert hasattr(L, name), name
 
 
+def test_binding_constants_match_the_header():
+    # the ctypes binding's option keys and tally indices are the header's
+    txt = open(os.path.join(ROOT, "include", "lightprop.h")).read()
+    opts = {m.group(1).lower(): int(m.group(2)) for m in re.finditer(r"#define LP_OPT_(\w+) (\d+)", txt)}
+    assert opts == lp._capi.OPTIONS
+    tallies = {m.group(1).lower(): int(m.group(2)) for m in re.finditer(r"#define LP_TALLY_(\w+) (\d+)", txt)}
+    assert tallies == lp.TALLY
+    assert int(re.search(r"#define LP_NTALLY (\d+)", txt).group(1)) == len(lp.TALLY)
+    assert len(lp.MISC) <= 32 and len(set(lp.MISC)) == len(lp.MISC)
+
+
 def test_create_fails_cleanly_or_succeeds():
     # without a device lp_create returns an error code (raised here); with
     # one it succeeds -- either way nothing crashes
